@@ -1,0 +1,586 @@
+// gfx950 kernels of the 3D-RISM residual F(t) = (chi - 1) * c(t) - t and
+// its setup / post-processing.  One residual evaluation is five launches:
+//
+//   1. closure_r2c_x : c = closure(beta v, t) fused into the x-axis r2c
+//   2. fft_y         : forward c2c along y
+//   3. zstar         : forward c2c along z, per-k m x m chi contraction,
+//                      tau (or Ng long-range) term, inverse c2c along z
+//   4. fft_y         : inverse c2c along y
+//   5. c2r_x_resid   : x-axis c2r fused with r = t' - t and its norms
+//
+// References: iterate_t1 hnc3d.c:1184-1285, star hnc3d.c:1115-1138,
+// compute_c hnc3d.c:167-181, closures.f90:107-128 and 248-432,
+// interp bgy3d-vec.c:23-46, KFREQ bgy3d.h:162, FFT layout
+// bgy3d-fftw3.c:364-385.
+#include "fft_device.hpp"
+#include "kernels.hpp"
+
+namespace rism {
+
+// ---------------------------------------------------------------- closures
+template <typename T> __device__ __forceinline__ T d_expm1(T x);
+template <> __device__ __forceinline__ double d_expm1<double>(double x) { return expm1(x); }
+template <> __device__ __forceinline__ float d_expm1<float>(float x) { return expm1f(x); }
+template <typename T> __device__ __forceinline__ T d_exp(T x);
+template <> __device__ __forceinline__ double d_exp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float d_exp<float>(float x) { return expf(x); }
+
+// Truncated exponential series sum_{k=1..n} x^k/k! by Horner
+// (closures.f90:191-236).
+template <typename T> __device__ __forceinline__ T exps1(int n, T x) {
+  T c = T(1), y = T(0);
+  for (int k = n; k >= 1; --k) { y = x * y + c; c = c * T(k); }
+  return (x / c) * y;
+}
+
+// c = closure(beta v, t) (closures.f90:107-128).  bv = beta * v.
+template <typename T> __device__ __forceinline__ T closure_c(int method, int order, T bv, T t) {
+  const T x = -bv + t;
+  switch (method) {
+    case 0:  // HNC
+      return d_expm1(x) - t;
+    case 1:  // KH
+      return (x <= T(0)) ? d_expm1(x) - t : -bv;
+    case 2:  // PY
+      return d_exp(-bv) * (T(1) + t) - T(1) - t;
+    default:  // PSE-n
+      return ((x < T(0)) ? d_expm1(x) : exps1(order, x)) - t;
+  }
+}
+
+// Chemical-potential density form(x, h, c, cl) (closures.f90:574-612).
+__device__ __forceinline__ double form_mu(int method, int order, double x, double h, double c,
+                                          double cl) {
+  const double base = -c - h * (c + cl) / 2;
+  switch (method) {
+    case 0: return base + h * h / 2;
+    case 1: return (h < 0) ? base + h * h / 2 : base;
+    case 2: return base;
+    default: {
+      if (h < 0) return base + h * h / 2;
+      const int m = order + 1;
+      double mx = 1, xm = 1;
+      for (int k = 1; k <= m; ++k) { mx *= k; xm *= x; }
+      return base + (h * h / 2 - xm / mx);
+    }
+  }
+}
+
+// ------------------------------------------------------------- row sizes
+__host__ __device__ constexpr int rows_for(int M) {
+  return M <= 16 ? 64 : M <= 32 ? 32 : M <= 64 ? 16 : M <= 128 ? 8 : 4;
+}
+
+// ------------------------------------------- 1. closure fused with x r2c
+// Grid: x = ceil(N1 N2 / ROWS), y = site, z = solute.
+template <typename T, int N0>
+__global__ void __launch_bounds__(NT)
+closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
+              const cpx<T>* __restrict__ tw_half, const cpx<T>* __restrict__ tw_full) {
+  constexpr int M = N0 / 2;
+  constexpr int ROWS = rows_for(M);
+  __shared__ cpx<T> lds[ROWS * M];
+  const SoluteDesc d = sd[blockIdx.z];
+  if (!d.active) return;
+  const int site = blockIdx.y;
+  const int nrows = g.N1 * g.N2;
+  const int row0 = blockIdx.x * ROWS;
+  const double* __restrict__ t = d.t + site * g.P + (int64_t)row0 * N0;
+  const T* __restrict__ bv = static_cast<const T*>(d.bv) + site * g.P + (int64_t)row0 * N0;
+  T* ldsr = reinterpret_cast<T*>(lds);
+  const int nvalid = min(ROWS, nrows - row0) * N0;
+  for (int e = threadIdx.x; e < ROWS * N0; e += NT) {
+    T c = T(0);
+    if (e < nvalid) c = closure_c<T>(g.closure, g.pse_order, bv[e], T(t[e]));
+    ldsr[e] = c;  // z_n = c_{2n} + i c_{2n+1}, row stride M complex
+  }
+  __syncthreads();
+  fft_lines<T, M, ROWS, NT, false>(lds, M, 1, tw_half);
+  cpx<T>* W = static_cast<cpx<T>*>(d.W) + site * g.K + (int64_t)row0 * g.NH;
+  const int nout = min(ROWS, nrows - row0) * (M + 1);
+  for (int e = threadIdx.x; e < nout; e += NT) {
+    const int row = e / (M + 1);
+    const int k = e - row * (M + 1);
+    const cpx<T> zk = lds[row * M + (k == M ? 0 : k)];
+    const cpx<T> zm = cconj(lds[row * M + (k == 0 ? 0 : M - k)]);
+    const cpx<T> s = cadd(zk, zm);
+    const cpx<T> dd = rot90<T, false>(cmul(tw_full[k], csub(zk, zm)));  // -i w (Zk - conj Zm)
+    W[row * g.NH + k] = cpx<T>{T(0.5) * (s.re + dd.re), T(0.5) * (s.im + dd.im)};
+  }
+}
+
+// -------------------------------------------------- 2/4. c2c along y
+// Grid: x = ceil(NH / LINES), y = site * N2 + z, z = solute.
+template <typename T, int N1, int LINES, bool INV>
+__global__ void __launch_bounds__(NT)
+fft_y(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw) {
+  constexpr int LS = N1 + 1;
+  __shared__ cpx<T> lds[LINES * LS];
+  const SoluteDesc d = sd[blockIdx.z];
+  if (!d.active) return;
+  const int site = blockIdx.y / g.N2;
+  const int z = blockIdx.y - site * g.N2;
+  const int kx0 = blockIdx.x * LINES;
+  cpx<T>* W = static_cast<cpx<T>*>(d.W) + site * g.K + (int64_t)z * g.N1 * g.NH + kx0;
+  for (int e = threadIdx.x; e < N1 * LINES; e += NT) {
+    const int y = e / LINES, l = e - y * LINES;
+    lds[l * LS + y] = (kx0 + l < g.NH) ? W[y * g.NH + l] : cpx<T>{T(0), T(0)};
+  }
+  __syncthreads();
+  fft_lines<T, N1, LINES, NT, INV>(lds, LS, 1, tw);
+  for (int e = threadIdx.x; e < N1 * LINES; e += NT) {
+    const int y = e / LINES, l = e - y * LINES;
+    if (kx0 + l < g.NH) W[y * g.NH + l] = lds[l * LS + y];
+  }
+}
+
+// 1D table lookup (interp, bgy3d-vec.c:23-46) at |k|; the index set is
+// shared by every table at one k-point.
+struct Interp {
+  int i, ip, im;
+  double off;  // ix - i
+};
+__device__ __forceinline__ Interp interp_at(double k, double dk, int n) {
+  const double ix = k / dk - 0.5;
+  int i = (int)(ix + 0.5);
+  i = i < 0 ? 0 : (i > n - 1 ? n - 1 : i);  // host guarantees range; clamp never faults
+  Interp r;
+  r.i = i;
+  r.ip = (i < n - 2) ? i + 1 : i;
+  r.im = (i > 1) ? i - 1 : i;
+  r.off = ix - i;
+  return r;
+}
+__device__ __forceinline__ double interp_eval(const Interp& a, const double* __restrict__ tab) {
+  const double fi = (tab[a.ip] - tab[a.im]) / (double)(a.ip - a.im);
+  return tab[a.i] + fi * a.off;
+}
+
+// --------------------------- 3. z c2c + chi contraction + tau + z c2c^-1
+// Grid: x = ceil(NH / LINES), y = ky row, z = solute.  Dynamic LDS:
+// m * LINES * (N2 + 1) complex.
+// chi_tab: [m(m+1)/2][nrad] chi_ij - delta_ij; tau_tab: [m][nrad] s_i(k).
+template <typename T, int N2, int LINES>
+__global__ void __launch_bounds__(NT)
+zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw,
+      const double* __restrict__ chi_tab, const double* __restrict__ tau_tab) {
+  constexpr int LS = N2 + 1;
+  extern __shared__ unsigned char smem[];
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+  const SoluteDesc d = sd[blockIdx.z];
+  if (!d.active) return;
+  const int m = g.m;
+  const int y = blockIdx.y;
+  const int kx0 = blockIdx.x * LINES;
+  const int64_t zs = (int64_t)g.N1 * g.NH;  // z stride
+  cpx<T>* W = static_cast<cpx<T>*>(d.W) + (int64_t)y * g.NH + kx0;
+  for (int s = 0; s < m; ++s) {
+    const cpx<T>* Ws = W + s * g.K;
+    cpx<T>* L = lds + s * LINES * LS;
+    for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
+      const int z = e / LINES, l = e - z * LINES;
+      L[l * LS + z] = (kx0 + l < g.NH) ? Ws[z * zs + l] : cpx<T>{T(0), T(0)};
+    }
+  }
+  __syncthreads();
+  for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, false>(lds + s * LINES * LS, LS, 1, tw);
+
+  const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + (int64_t)y * g.NH + kx0;
+  const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
+  const double ky = iy * g.dky;
+  for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
+    const int z = e / LINES, l = e - z * LINES;
+    if (kx0 + l >= g.NH) continue;
+    const int iz = (z <= N2 / 2) ? z : z - N2;
+    const double kx = (kx0 + l) * g.dkx, kz = iz * g.dkz;
+    const double k = sqrt(kx * kx + ky * ky + kz * kz);
+    const Interp ip = interp_at(k, g.dk_tab, g.nrad);
+    const cpx<T> fs = Fs[z * zs + l];
+    cpx<T> c[MAX_SITES];
+    for (int s = 0; s < m; ++s) c[s] = lds[(s * LINES + l) * LS + z];
+    cpx<T> uc = {T(0), T(0)};
+    if (!g.renorm) {
+      // u_c(k) = 4 pi eps0^-1 exp(-k^2/4G^2)/k^2 F(k) s(k), 0 at k = 0
+      // (bgy3d-force.h:140-147, bgy3d-solutes.c:767-792).
+      const double k2 = kx * kx + ky * ky + kz * kz;
+      const double shape = (k2 == 0.0) ? 0.0
+          : (4 * M_PI * EPSILON0INV) * exp(-k2 / (4 * G_COULOMB * G_COULOMB)) / k2;
+      uc = cscale(fs, T(shape));
+      for (int s = 0; s < m; ++s) {
+        // h^3 C_j - beta q_j u_c  (hnc3d.c:1226-1227)
+        c[s] = csub(cscale(c[s], T(g.h3)), cscale(uc, T(g.beta * g.q[s])));
+      }
+    }
+    for (int i = 0; i < m; ++i) {
+      cpx<T> acc = {T(0), T(0)};
+      for (int j = 0; j < m; ++j) {
+        const int pi_ = (i >= j) ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+        const T chi = T(interp_eval(ip, chi_tab + (int64_t)pi_ * g.nrad));
+        acc.re += chi * c[j].re;
+        acc.im += chi * c[j].im;
+      }
+      cpx<T> out;
+      if (g.renorm) {
+        // (h^3/L^3) sum_j chi_ij C_j - (beta/eps0/L^3) tau_i F s  (hnc3d.c:1245-1268)
+        const T tau = T(interp_eval(ip, tau_tab + (int64_t)i * g.nrad) * g.tau_scale);
+        out = csub(cscale(acc, T(g.a_scale)), cscale(fs, tau));
+      } else {
+        out = cscale(csub(acc, cscale(uc, T(g.beta * g.q[i]))), T(g.inv_L3));
+      }
+      lds[(i * LINES + l) * LS + z] = out;
+    }
+  }
+  __syncthreads();
+  for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, true>(lds + s * LINES * LS, LS, 1, tw);
+  for (int s = 0; s < m; ++s) {
+    cpx<T>* Ws = W + s * g.K;
+    const cpx<T>* L = lds + s * LINES * LS;
+    for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
+      const int z = e / LINES, l = e - z * LINES;
+      if (kx0 + l < g.NH) Ws[z * zs + l] = L[l * LS + z];
+    }
+  }
+}
+
+// ------------------------------------------------ block reductions
+__device__ __forceinline__ double warp_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double warp_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// -------------------------------------- 5. x c2r fused with the residual
+// Grid as closure_r2c_x.  Writes r = t' - t and (max|r|, sum r^2) per
+// workgroup to part[(site * gridDim.x + blockIdx.x) * 2 + {0,1}].
+template <typename T, int N0>
+__global__ void __launch_bounds__(NT)
+c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
+            const cpx<T>* __restrict__ tw_half, const cpx<T>* __restrict__ tw_full) {
+  constexpr int M = N0 / 2;
+  constexpr int ROWS = rows_for(M);
+  __shared__ cpx<T> xb[ROWS * (M + 1)];
+  __shared__ cpx<T> zb[ROWS * M];
+  __shared__ double red[2][NT / 64];
+  const SoluteDesc d = sd[blockIdx.z];
+  if (!d.active) return;
+  const int site = blockIdx.y;
+  const int nrows = g.N1 * g.N2;
+  const int row0 = blockIdx.x * ROWS;
+  const int vrows = min(ROWS, nrows - row0);
+  const cpx<T>* W = static_cast<const cpx<T>*>(d.W) + site * g.K + (int64_t)row0 * g.NH;
+  for (int e = threadIdx.x; e < ROWS * (M + 1); e += NT) {
+    const int row = e / (M + 1);
+    const int k = e - row * (M + 1);
+    xb[e] = (row < vrows) ? W[row * g.NH + k] : cpx<T>{T(0), T(0)};
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ROWS * M; e += NT) {
+    const int row = e / M;
+    const int k = e - row * M;
+    cpx<T> xk = xb[row * (M + 1) + k];
+    cpx<T> xm = xb[row * (M + 1) + (M - k)];
+    if (k == 0) { xk.im = T(0); xm.im = T(0); }  // c2r ignores Im at DC / Nyquist
+    const cpx<T> a = cadd(xk, cconj(xm));
+    const cpx<T> b = csub(xk, cconj(xm));
+    const cpx<T> ib = rot90<T, true>(cmulc(b, tw_full[k]));  // i e^{+2 pi i k/N} b
+    zb[e] = cadd(a, ib);
+  }
+  __syncthreads();
+  fft_lines<T, M, ROWS, NT, true>(zb, M, 1, tw_half);
+  const T* zr = reinterpret_cast<const T*>(zb);
+  const double* __restrict__ t = d.t + site * g.P + (int64_t)row0 * N0;
+  double* __restrict__ r = d.r + site * g.P + (int64_t)row0 * N0;
+  double mx = 0.0, ss = 0.0;
+  for (int e = threadIdx.x; e < vrows * N0; e += NT) {
+    const double rv = (double)zr[e] - t[e];
+    r[e] = rv;
+    mx = fmax(mx, fabs(rv));
+    ss += rv * rv;
+  }
+  mx = warp_max(mx);
+  ss = warp_sum(ss);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) { red[0][w] = mx; red[1][w] = ss; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = red[0][0], b = red[1][0];
+    for (int i = 1; i < NT / 64; ++i) { a = fmax(a, red[0][i]); b += red[1][i]; }
+    double* p = d.part + ((int64_t)site * gridDim.x + blockIdx.x) * 2;
+    p[0] = a;
+    p[1] = b;
+  }
+}
+
+// Sum the residual partials of each solute in a fixed order.
+// Grid: x = solute.  out[b*2 + {0,1}] = (max|r|, ||r||_2^2).
+__global__ void __launch_bounds__(NT)
+reduce_resid(const SoluteDesc* __restrict__ sd, int nparts, double* __restrict__ out) {
+  __shared__ double red[2][NT / 64];
+  const SoluteDesc d = sd[blockIdx.x];
+  if (!d.active) return;
+  double mx = 0.0, ss = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += NT) {
+    mx = fmax(mx, d.part[2 * i]);
+    ss += d.part[2 * i + 1];
+  }
+  mx = warp_max(mx);
+  ss = warp_sum(ss);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) { red[0][w] = mx; red[1][w] = ss; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = red[0][0], b = red[1][0];
+    for (int i = 1; i < NT / 64; ++i) { a = fmax(a, red[0][i]); b += red[1][i]; }
+    // a NaN anywhere must surface: fmax drops NaNs, the sum keeps them
+    out[2 * blockIdx.x] = (b == b) ? a : b;
+    out[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// ----------------------------------------------------------- MDIIS
+// dots[b][j] = <rnew, rh[j]> over m P elements.  Grid: x = chunks,
+// y = solute.  Partials to part[(b * gridDim.x + x) * MAX_DIIS + j].
+__global__ void __launch_bounds__(NT)
+diis_dots(const DiisDesc* __restrict__ dd, int64_t n, double* __restrict__ part) {
+  __shared__ double red[MAX_DIIS][NT / 64];
+  const DiisDesc d = dd[blockIdx.y];
+  if (!d.active) return;
+  double acc[MAX_DIIS];
+#pragma unroll
+  for (int j = 0; j < MAX_DIIS; ++j) acc[j] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) {
+    const double a = d.rnew[e];
+#pragma unroll
+    for (int j = 0; j < MAX_DIIS; ++j)
+      if (j < d.nh) acc[j] += a * d.rh[j][e];
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < MAX_DIIS; ++j) {
+    if (j < d.nh) {
+      const double s = warp_sum(acc[j]);
+      if (lane == 0) red[j][w] = s;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < d.nh) {
+    double s = red[threadIdx.x][0];
+    for (int i = 1; i < NT / 64; ++i) s += red[threadIdx.x][i];
+    part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * MAX_DIIS + threadIdx.x] = s;
+  }
+}
+
+// Fixed-order sum of the dot partials.  Grid: x = solute.
+__global__ void __launch_bounds__(NT)
+diis_dots_reduce(const DiisDesc* __restrict__ dd, int nchunks, const double* __restrict__ part,
+                 double* __restrict__ out) {
+  const DiisDesc d = dd[blockIdx.x];
+  if (!d.active) return;
+  const int j = threadIdx.x;
+  if (j < d.nh) {
+    double s = 0.0;
+    const double* p = part + (int64_t)blockIdx.x * nchunks * MAX_DIIS + j;
+    for (int c = 0; c < nchunks; ++c) s += p[(int64_t)c * MAX_DIIS];
+    out[blockIdx.x * MAX_DIIS + j] = s;
+  }
+}
+
+// tout = sum_j coef_j (th_j + lam rh_j).  Grid: x = chunks, y = solute.
+__global__ void __launch_bounds__(NT)
+diis_update(const DiisDesc* __restrict__ dd, int64_t n) {
+  const DiisDesc d = dd[blockIdx.y];
+  if (!d.active) return;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAX_DIIS; ++j)
+      if (j < d.nh) s += d.coef[j] * (d.th[j][e] + d.lam * d.rh[j][e]);
+    d.tout[e] = s;
+  }
+}
+
+// ------------------------------------------------------ setup kernels
+// Capped LJ and short-range Coulomb (bgy3d-solutes.c:45-131, 196-286).
+__device__ __forceinline__ double lj0(double r) {
+  const double p2 = 1.0 / (r * r);
+  const double p6 = p2 * p2 * p2;
+  return 4 * p6 * (p6 - 1);
+}
+__device__ __forceinline__ double lj1(double r) {
+  const double p2 = 1.0 / (r * r);
+  const double p6 = p2 * p2 * p2;
+  return 4 * p6 * (6 - 12 * p6) / r;
+}
+__device__ __forceinline__ double ljcap0(double r) {
+  const double R = 0.2;
+  if (r >= R) return lj0(r);
+  const double b = lj1(R) / (2 * R), a = lj0(R);
+  return a + b * (r - R) * (r + R);
+}
+__device__ __forceinline__ double cs0(double r) { return erfc(r) / r; }
+__device__ __forceinline__ double cs1(double r) {
+  return -(erfc(r) + (2 / sqrt(M_PI)) * r * exp(-r * r)) / (r * r);
+}
+__device__ __forceinline__ double cscap0(double r) {
+  const double R = 0.2;
+  if (r >= R) return cs0(r);
+  const double b = cs1(R) / (2 * R), a = cs0(R);
+  return a + b * (r - R) * (r + R);
+}
+// erf(r)/r with its Taylor form below 0.02 (bgy3d-solutes.c:148-159)
+__device__ __forceinline__ double cl0(double r) {
+  if (r >= 0.02) return erf(r) / r;
+  const double r2 = r * r;
+  return (2 / sqrt(M_PI)) * (1. + r2 * (-1. / 3. + r2 * (1. / 10. + r2 * (-1. / 42.))));
+}
+
+// beta v_i(r) for every solvent site (field0, bgy3d-solutes.c:344-377).
+// Grid: x = chunks of P, y = solute.  Writes FP64 and, if bv32 != NULL,
+// an FP32 copy for the mixed-precision bulk iterations.
+__global__ void __launch_bounds__(NT)
+field_bv(const SitesDesc* __restrict__ sites, GridArgs g, const double* __restrict__ solvent_sse,
+         double L0, double L1, double L2, double h0, double h1, double h2,
+         double* const* __restrict__ bv64, float* const* __restrict__ bv32) {
+  const SitesDesc sd = sites[blockIdx.y];
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.P; e += stride) {
+    const int x = (int)(e % g.N0);
+    const int y = (int)((e / g.N0) % g.N1);
+    const int z = (int)(e / ((int64_t)g.N0 * g.N1));
+    const double rx = x * h0 - L0 / 2, ry = y * h1 - L1 / 2, rz = z * h2 - L2 / 2;
+    double v[MAX_SITES];
+    for (int i = 0; i < g.m; ++i) v[i] = 0.0;
+    for (int a = 0; a < sd.n; ++a) {
+      const double* s = sd.xyz_s_e_q + 6 * a;
+      const double dx = rx - s[0], dy = ry - s[1], dz = rz - s[2];
+      const double rab = sqrt(dx * dx + dy * dy + dz * dz);
+      const double csc = G_COULOMB * cscap0(G_COULOMB * rab);
+      for (int i = 0; i < g.m; ++i) {
+        const double* sv = solvent_sse + 3 * i;  // sigma, epsilon, charge
+        const double eab = sqrt(sv[1] * s[4]);
+        const double sab = 0.5 * (sv[0] + s[3]);
+        const double qab = sv[2] * s[5] * EPSILON0INV;
+        v[i] += eab * ljcap0(rab / sab) + qab * csc;
+      }
+    }
+    for (int i = 0; i < g.m; ++i) {
+      const double b = g.beta * v[i];
+      bv64[blockIdx.y][i * g.P + e] = b;
+      if (bv32) bv32[blockIdx.y][i * g.P + e] = (float)b;
+    }
+  }
+}
+
+// F(k) s(k) = s(k) sum_a Q_a exp(-i k.x_a), corner k (form_factor,
+// bgy3d-solutes.c:587-600, times bgy3d_vec_fft_trans bgy3d-vec.c:619-646).
+// Grid: x = chunks of K, y = solute.
+__global__ void __launch_bounds__(NT)
+form_factor(const SitesDesc* __restrict__ sites, GridArgs g, cpx<double>* const* __restrict__ fs64,
+            cpx<float>* const* __restrict__ fs32) {
+  const SitesDesc sd = sites[blockIdx.y];
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.K; e += stride) {
+    const int x = (int)(e % g.NH);
+    const int y = (int)((e / g.NH) % g.N1);
+    const int z = (int)(e / ((int64_t)g.NH * g.N1));
+    const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
+    const int iz = (z <= g.N2 / 2) ? z : z - g.N2;
+    const double kx = x * g.dkx, ky = iy * g.dky, kz = iz * g.dkz;
+    double re = 0.0, im = 0.0;
+    for (int a = 0; a < sd.n; ++a) {
+      const double* s = sd.xyz_s_e_q + 6 * a;
+      double sn, cs;
+      sincos(kx * s[0] + ky * s[1] + kz * s[2], &sn, &cs);
+      re += s[5] * cs;
+      im -= s[5] * sn;
+    }
+    const double sign = ((x + iy + iz) & 1) ? -1.0 : 1.0;
+    fs64[blockIdx.y][e] = cpx<double>{sign * re, sign * im};
+    if (fs32) fs32[blockIdx.y][e] = cpx<float>{(float)(sign * re), (float)(sign * im)};
+  }
+}
+
+// -------------------------------------------------- mu and observables
+// Per point: c = closure(beta v, t), h = c + t, x = -beta v + t,
+// cl_i = -beta q_i u_c(r) with u_c = eps0^-1 sum_a Q_a G cl0(G r)
+// (coulomb_long, bgy3d-solutes.c:796-815); accumulate the chemical
+// potential densities of the native closure, HNC, KH, GF and sum c
+// (hnc3d.c:2172-2268), with Neumaier-compensated FP64 sums.  Writes
+// g = h + 1 if gout != NULL.
+// Grid: x = chunks of P, y = solute.  part[(b*gridDim.x + x)*5 + q].
+__global__ void __launch_bounds__(NT)
+mu_density(const SitesDesc* __restrict__ sites, GridArgs g, const double* const* __restrict__ tt,
+           const double* const* __restrict__ bv64, double L0, double L1, double L2, double h0,
+           double h1, double h2, double* const* __restrict__ gout, double* __restrict__ part) {
+  __shared__ double red[5][NT / 64];
+  const SitesDesc sd = sites[blockIdx.y];
+  const double* t = tt[blockIdx.y];
+  const double* bv = bv64[blockIdx.y];
+  double* go = gout ? gout[blockIdx.y] : nullptr;
+  double sum[5] = {0, 0, 0, 0, 0}, comp[5] = {0, 0, 0, 0, 0};
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.P; e += stride) {
+    const int x = (int)(e % g.N0);
+    const int y = (int)((e / g.N0) % g.N1);
+    const int z = (int)(e / ((int64_t)g.N0 * g.N1));
+    const double rx = x * h0 - L0 / 2, ry = y * h1 - L1 / 2, rz = z * h2 - L2 / 2;
+    double uc = 0.0;
+    for (int a = 0; a < sd.n; ++a) {
+      const double* s = sd.xyz_s_e_q + 6 * a;
+      const double dx = rx - s[0], dy = ry - s[1], dz = rz - s[2];
+      uc += s[5] * G_COULOMB * cl0(G_COULOMB * sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    uc *= EPSILON0INV;
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < g.m; ++i) {
+      const int64_t idx = i * g.P + e;
+      const double ti = t[idx], bvi = bv[idx];
+      const double c = closure_c<double>(g.closure, g.pse_order, bvi, ti);
+      const double h = c + ti;
+      const double xx = -bvi + ti;
+      const double cl = -g.beta * g.q[i] * uc;
+      v[0] += form_mu(g.closure, g.pse_order, xx, h, c, cl);
+      v[1] += form_mu(0, 0, xx, h, c, cl);
+      v[2] += form_mu(1, 1, xx, h, c, cl);
+      v[3] += form_mu(2, 0, xx, h, c, cl);
+      v[4] += c;
+      if (go) go[idx] = h + 1.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {  // Neumaier
+      const double s = sum[q] + v[q];
+      comp[q] += (fabs(sum[q]) >= fabs(v[q])) ? (sum[q] - s) + v[q] : (v[q] - s) + sum[q];
+      sum[q] = s;
+    }
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const double s = warp_sum(sum[q] + comp[q]);
+    if (lane == 0) red[q][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double s = red[threadIdx.x][0];
+    for (int i = 1; i < NT / 64; ++i) s += red[threadIdx.x][i];
+    part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 5 + threadIdx.x] = s;
+  }
+}
+
+// Picard step t += lam r, or copy.  Grid: x = chunks, y = solute.
+__global__ void __launch_bounds__(NT)
+axpy_batch(double* const* __restrict__ y, const double* const* __restrict__ x, double a,
+           int64_t n, const int* __restrict__ active) {
+  if (active && !active[blockIdx.y]) return;
+  double* yy = y[blockIdx.y];
+  const double* xx = x[blockIdx.y];
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) yy[e] += a * xx[e];
+}
+
+}  // namespace rism

@@ -1,0 +1,897 @@
+// Host side of the MI355X 3D-RISM solver: workspace, kernel dispatch,
+// the batched MDIIS / Picard loop, and the C ABI of include/rism3d.h.
+//
+// Reference counterparts: hnc3d_solute_solve (hnc3d.c:1762-2350),
+// solvent_kernel_rism (hnc3d.c:1443-1566), rism_solvent_renorm
+// (rism.f90:141-208), chempot / pmv_factor (hnc3d.c:604-737),
+// bgy3d_snes_picard (bgy3d-snes.c:348-384).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rism3d.h"
+
+#include "kernels.hip"
+
+namespace rism {
+namespace {
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) {                                                              \
+      std::fprintf(stderr, "rism3d: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), \
+                   __FILE__, __LINE__);                                                  \
+      throw HipFailure{e_};                                                              \
+    }                                                                                    \
+  } while (0)
+
+struct HipFailure {
+  hipError_t err;
+};
+
+bool supported_n(int n) {
+  switch (n) {
+    case 32: case 48: case 64: case 96: case 128: case 192: case 256: case 384: return true;
+    default: return false;
+  }
+}
+
+// ----------------------------------------------------- kernel dispatch
+#define RISM_FOR_N(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384)
+
+template <typename T>
+void launch_closure_r2c(int N0, dim3 grid, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
+                        const cpx<T>* twh, const cpx<T>* twf) {
+  switch (N0) {
+#define X(n) case n: hipLaunchKernelGGL((closure_r2c_x<T, n>), grid, dim3(NT), 0, s, sd, g, twh, twf); break;
+    RISM_FOR_N(X)
+#undef X
+  }
+}
+template <typename T>
+void launch_c2r_resid(int N0, dim3 grid, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
+                      const cpx<T>* twh, const cpx<T>* twf) {
+  switch (N0) {
+#define X(n) case n: hipLaunchKernelGGL((c2r_x_resid<T, n>), grid, dim3(NT), 0, s, sd, g, twh, twf); break;
+    RISM_FOR_N(X)
+#undef X
+  }
+}
+template <typename T> constexpr int y_lines() { return 128 / (int)sizeof(cpx<T>); }
+template <typename T, bool INV>
+void launch_fft_y(int N1, dim3 grid, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
+                  const cpx<T>* tw) {
+  constexpr int LN = y_lines<T>();
+  switch (N1) {
+#define X(n) case n: hipLaunchKernelGGL((fft_y<T, n, LN, INV>), grid, dim3(NT), 0, s, sd, g, tw); break;
+    RISM_FOR_N(X)
+#undef X
+  }
+}
+template <typename T, int LINES>
+void launch_zstar_l(int N2, dim3 grid, size_t lds, hipStream_t s, const SoluteDesc* sd,
+                    const GridArgs& g, const cpx<T>* tw, const double* chi, const double* tau) {
+  switch (N2) {
+#define X(n) case n: hipLaunchKernelGGL((zstar<T, n, LINES>), grid, dim3(NT), lds, s, sd, g, tw, chi, tau); break;
+    RISM_FOR_N(X)
+#undef X
+  }
+}
+template <typename T>
+int zstar_lines(int N2, int m) {
+  int L = (sizeof(T) == 8) ? 8 : 16;
+  while (L > 2 && (size_t)m * L * (N2 + 1) * sizeof(cpx<T>) > 64 * 1024) L /= 2;
+  return L;
+}
+template <typename T>
+void launch_zstar(int N2, int NH, int N1, int B, hipStream_t s, const SoluteDesc* sd,
+                  const GridArgs& g, const cpx<T>* tw, const double* chi, const double* tau) {
+  const int L = zstar_lines<T>(N2, g.m);
+  const size_t lds = (size_t)g.m * L * (N2 + 1) * sizeof(cpx<T>);
+  dim3 grid((NH + L - 1) / L, N1, B);
+  switch (L) {
+    case 2: launch_zstar_l<T, 2>(N2, grid, lds, s, sd, g, tw, chi, tau); break;
+    case 4: launch_zstar_l<T, 4>(N2, grid, lds, s, sd, g, tw, chi, tau); break;
+    case 8: launch_zstar_l<T, 8>(N2, grid, lds, s, sd, g, tw, chi, tau); break;
+    default: launch_zstar_l<T, 16>(N2, grid, lds, s, sd, g, tw, chi, tau); break;
+  }
+}
+
+// ------------------------------------------------------ device buffers
+template <typename X>
+struct DevBuf {
+  X* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) : n(count) {
+    if (count) HIPCHK(hipMalloc(&p, count * sizeof(X)));
+  }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+template <typename T>
+std::vector<cpx<T>> twiddles(int n) {
+  std::vector<cpx<T>> w(n);
+  for (int j = 0; j < n; ++j) {
+    const long double a = -2.0L * 3.14159265358979323846264338327950288L * j / n;
+    w[j] = cpx<T>{(T)std::cos(a), (T)std::sin(a)};
+  }
+  return w;
+}
+
+struct Twiddles {
+  DevBuf<cpx<double>> d[4];  // N0, N0/2, N1, N2
+  DevBuf<cpx<float>> f[4];
+};
+
+void make_twiddles(Twiddles& tw, const int N[3], hipStream_t s) {
+  const int len[4] = {N[0], N[0] / 2, N[1], N[2]};
+  for (int i = 0; i < 4; ++i) {
+    auto wd = twiddles<double>(len[i]);
+    auto wf = twiddles<float>(len[i]);
+    tw.d[i] = DevBuf<cpx<double>>(len[i]);
+    tw.f[i] = DevBuf<cpx<float>>(len[i]);
+    HIPCHK(hipMemcpyAsync(tw.d[i].p, wd.data(), wd.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(tw.f[i].p, wf.data(), wf.size() * sizeof(cpx<float>), hipMemcpyHostToDevice, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------ DIIS solve
+// Minimise |sum c_j r_j|^2 subject to sum c_j = 1 (Pulay).  G is the Gram
+// matrix over the nh history entries (in order).  Returns false if the
+// bordered system is singular.
+bool diis_coefficients(const std::vector<double>& G, int nh, std::vector<double>& c) {
+  const int n = nh + 1;
+  std::vector<long double> A(n * n, 0.0L), b(n, 0.0L);
+  long double scale = 0.0L;
+  for (int i = 0; i < nh; ++i) scale = std::max(scale, (long double)std::fabs(G[i * nh + i]));
+  if (!(scale > 0.0L) || !std::isfinite((double)scale)) return false;
+  for (int i = 0; i < nh; ++i) {
+    for (int j = 0; j < nh; ++j) A[i * n + j] = G[i * nh + j] / scale;
+    A[i * n + nh] = -1.0L;
+    A[nh * n + i] = -1.0L;
+  }
+  b[nh] = -1.0L;
+  for (int col = 0; col < n; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < n; ++r)
+      if (std::fabs((double)A[r * n + col]) > std::fabs((double)A[piv * n + col])) piv = r;
+    if (std::fabs((double)A[piv * n + col]) < 1e-300) return false;
+    if (piv != col) {
+      for (int k = 0; k < n; ++k) std::swap(A[col * n + k], A[piv * n + k]);
+      std::swap(b[col], b[piv]);
+    }
+    for (int r = col + 1; r < n; ++r) {
+      const long double f = A[r * n + col] / A[col * n + col];
+      for (int k = col; k < n; ++k) A[r * n + k] -= f * A[col * n + k];
+      b[r] -= f * b[col];
+    }
+  }
+  std::vector<long double> x(n);
+  for (int r = n - 1; r >= 0; --r) {
+    long double s = b[r];
+    for (int k = r + 1; k < n; ++k) s -= A[r * n + k] * x[k];
+    x[r] = s / A[r * n + r];
+  }
+  c.assign(nh, 0.0);
+  for (int i = 0; i < nh; ++i) {
+    c[i] = (double)x[i];
+    if (!std::isfinite(c[i])) return false;
+  }
+  return true;
+}
+
+// --------------------------------------------------------------- solver
+struct SoluteState {
+  // device memory, one slab each
+  DevBuf<double> th;       // D * mP  iterate history
+  DevBuf<double> rh;       // D * mP  residual history
+  DevBuf<double> bv64;     // mP
+  DevBuf<float> bv32;      // mP (mixed only)
+  DevBuf<cpx<double>> W;   // mK (fp64 view; fp32 view uses the front half)
+  DevBuf<cpx<double>> fs64;
+  DevBuf<cpx<float>> fs32;
+  DevBuf<double> part;     // residual partials
+  DevBuf<double> sites;    // n * 6
+  int n_sites = 0;
+  // host iteration state
+  std::vector<int> order;  // history slots, oldest first
+  std::vector<double> gram;  // D x D by slot
+  int cur = 0;             // slot of the iterate being evaluated
+  int active = 1;
+  int status = RISM3D_NONCONVERGED;
+  int iters = 0, iters32 = 0;
+  int phase32 = 0;         // 1 while the FP32 kernels run
+  int reached_mixed = 0;   // FP32 phase ended at mixed_tol
+  double resid = INFINITY, best = INFINITY;
+  int since_best = 0;
+};
+
+struct Problem {
+  GridArgs g;
+  int N[3];
+  double L[3], h[3];
+  int D;          // DIIS depth
+  int mixed;
+  int B;
+  int64_t mP;
+  hipStream_t s;
+  Twiddles tw;
+  DevBuf<double> chi_tab, tau_tab;
+  DevBuf<SoluteDesc> d_sd;
+  DevBuf<DiisDesc> d_dd;
+  DevBuf<SitesDesc> d_sites;
+  DevBuf<double> d_out;       // B * 2 residual norms
+  DevBuf<double> d_dotpart;   // B * nchunks * MAX_DIIS
+  DevBuf<double> d_dots;      // B * MAX_DIIS
+  int nchunks = 0;
+  int nparts = 0;
+  std::vector<SoluteState> st;
+};
+
+int resid_grid_x(const GridArgs& g) {
+  const int rows = rows_for(g.N0 / 2);
+  return (g.N1 * g.N2 + rows - 1) / rows;
+}
+
+void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
+  const GridArgs& g = pb.g;
+  HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), sd.size() * sizeof(SoluteDesc), hipMemcpyHostToDevice, pb.s));
+  const dim3 gx(resid_grid_x(g), g.m, pb.B);
+  if (fp32) {
+    using T = float;
+    constexpr int LN = y_lines<T>();
+    const dim3 gy((g.NH + LN - 1) / LN, g.m * g.N2, pb.B);
+    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p);
+    launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p);
+    launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi_tab.p, pb.tau_tab.p);
+    launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p);
+    launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p);
+  } else {
+    using T = double;
+    constexpr int LN = y_lines<T>();
+    const dim3 gy((g.NH + LN - 1) / LN, g.m * g.N2, pb.B);
+    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
+    launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
+    launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi_tab.p, pb.tau_tab.p);
+    launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
+    launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
+  }
+  hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, pb.d_sd.p, pb.nparts, pb.d_out.p);
+}
+
+struct Config {
+  int solver, precision, polish, depth, verbose, max_iter;
+  double lam, tol, mixed_tol;
+};
+
+// Runs the fixed-point iteration for every solute of the batch.
+void iterate_batch(Problem& pb, const Config& cf) {
+  const int B = pb.B;
+  const int64_t mP = pb.mP;
+  const int D = pb.D;
+  std::vector<SoluteDesc> sd(B);
+  std::vector<DiisDesc> dd(B);
+  std::vector<double> out(B * 2), dots(B * MAX_DIIS);
+  const bool mdiis = cf.solver == RISM3D_SOLVER_MDIIS;
+  for (int b = 0; b < B; ++b) {
+    auto& s = pb.st[b];
+    s.phase32 = cf.precision == RISM3D_MIXED;
+    s.gram.assign(D * D, 0.0);
+    s.order.clear();
+    s.cur = 0;
+  }
+  int polish_left = cf.polish;
+  const dim3 gchunk(pb.nchunks, B);
+  for (;;) {
+    bool any = false, any32 = false;
+    for (int b = 0; b < B; ++b) {
+      auto& s = pb.st[b];
+      if (s.active) { any = true; any32 |= (s.phase32 != 0); }
+    }
+    if (!any) break;
+    // All active solutes of a launch run one precision: FP32 while any
+    // still needs FP32 (the others wait), then FP64.
+    const bool fp32 = any32;
+    for (int b = 0; b < B; ++b) {
+      auto& s = pb.st[b];
+      const bool run = s.active && (s.phase32 != 0) == fp32;
+      sd[b].t = s.th.p + (int64_t)s.cur * mP;
+      sd[b].r = s.rh.p + (int64_t)s.cur * mP;
+      sd[b].bv = fp32 ? (const void*)s.bv32.p : (const void*)s.bv64.p;
+      sd[b].W = s.W.p;
+      sd[b].Fs = fp32 ? (const void*)s.fs32.p : (const void*)s.fs64.p;
+      sd[b].part = s.part.p;
+      sd[b].active = run;
+      // history including the slot being evaluated now
+      dd[b] = DiisDesc{};
+      dd[b].active = run && mdiis;
+      if (run && mdiis) {
+        std::vector<int> hist = s.order;
+        hist.erase(std::remove(hist.begin(), hist.end(), s.cur), hist.end());
+        hist.push_back(s.cur);
+        if ((int)hist.size() > D) hist.erase(hist.begin());
+        dd[b].rnew = s.rh.p + (int64_t)s.cur * mP;
+        dd[b].nh = (int)hist.size();
+        for (int j = 0; j < dd[b].nh; ++j) {
+          dd[b].rh[j] = s.rh.p + (int64_t)hist[j] * mP;
+          dd[b].th[j] = s.th.p + (int64_t)hist[j] * mP;
+        }
+      }
+    }
+    evaluate(pb, fp32, sd);
+    if (mdiis) {
+      HIPCHK(hipMemcpyAsync(pb.d_dd.p, dd.data(), B * sizeof(DiisDesc), hipMemcpyHostToDevice, pb.s));
+      hipLaunchKernelGGL(diis_dots, gchunk, dim3(NT), 0, pb.s, pb.d_dd.p, mP, pb.d_dotpart.p);
+      hipLaunchKernelGGL(diis_dots_reduce, dim3(B), dim3(NT), 0, pb.s, pb.d_dd.p, pb.nchunks,
+                         pb.d_dotpart.p, pb.d_dots.p);
+      HIPCHK(hipMemcpyAsync(dots.data(), pb.d_dots.p, B * MAX_DIIS * sizeof(double), hipMemcpyDeviceToHost, pb.s));
+    }
+    HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, B * 2 * sizeof(double), hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+    HIPCHK(hipGetLastError());
+
+    bool any_update = false;
+    for (int b = 0; b < B; ++b) {
+      auto& s = pb.st[b];
+      if (!sd[b].active) { dd[b].active = 0; continue; }
+      const double rinf = out[2 * b];
+      s.iters++;
+      if (fp32) s.iters32++;
+      s.resid = rinf;
+      if (cf.verbose)
+        std::fprintf(stderr, "rism3d: solute %d iter %d %s |F|inf = %.3e |F|2 = %.3e\n", b, s.iters,
+                     fp32 ? "fp32" : "fp64", rinf, std::sqrt(out[2 * b + 1]));
+      if (!std::isfinite(rinf)) {
+        s.status = RISM3D_NAN;
+        s.active = 0;
+        dd[b].active = 0;
+        continue;
+      }
+      if (fp32) {
+        if (rinf <= cf.mixed_tol || s.iters32 >= cf.max_iter) {
+          // hand over to FP64: restart the history at this iterate
+          s.reached_mixed = rinf <= cf.mixed_tol;
+          s.phase32 = 0;
+          s.order.clear();
+          dd[b].active = 0;
+          s.best = INFINITY;
+          s.since_best = 0;
+          continue;  // the same slot is re-evaluated in FP64
+        }
+      } else {
+        const bool polish_done = cf.precision == RISM3D_MIXED && polish_left <= 0;
+        if (rinf <= cf.tol || polish_done) {
+          s.status = (rinf <= cf.tol || (cf.precision == RISM3D_MIXED && s.reached_mixed))
+                         ? RISM3D_OK : RISM3D_NONCONVERGED;
+          s.active = 0;
+          dd[b].active = 0;
+          continue;
+        }
+        if (s.iters - s.iters32 >= cf.max_iter) {
+          s.status = RISM3D_NONCONVERGED;
+          s.active = 0;
+          dd[b].active = 0;
+          continue;
+        }
+      }
+      if (!mdiis) {
+        // Picard: t_next = t + lam r, in a fresh slot (bgy3d-snes.c:367-383)
+        const int next = (s.cur + 1) % D;
+        dd[b].active = 1;
+        dd[b].nh = 1;
+        dd[b].rh[0] = s.rh.p + (int64_t)s.cur * mP;
+        dd[b].th[0] = s.th.p + (int64_t)s.cur * mP;
+        dd[b].coef[0] = 1.0;
+        dd[b].lam = cf.lam;
+        dd[b].tout = s.th.p + (int64_t)next * mP;
+        s.cur = next;
+        any_update = true;
+        continue;
+      }
+      // MDIIS bookkeeping: enter the new slot, refresh its Gram row.
+      std::vector<int> hist = s.order;
+      hist.erase(std::remove(hist.begin(), hist.end(), s.cur), hist.end());
+      hist.push_back(s.cur);
+      if ((int)hist.size() > D) hist.erase(hist.begin());
+      for (int j = 0; j < (int)hist.size(); ++j) {
+        s.gram[s.cur * D + hist[j]] = dots[b * MAX_DIIS + j];
+        s.gram[hist[j] * D + s.cur] = dots[b * MAX_DIIS + j];
+      }
+      // restart when the residual blows up against the best one seen
+      const double r2 = out[2 * b + 1];
+      if (r2 < s.best) { s.best = r2; s.since_best = 0; } else { s.since_best++; }
+      if (r2 > 100.0 * s.best || s.since_best > 3 * D) {
+        hist.assign(1, s.cur);
+        s.best = r2;
+        s.since_best = 0;
+      }
+      s.order = hist;
+      const int nh = (int)hist.size();
+      std::vector<double> G(nh * nh), c;
+      for (int i = 0; i < nh; ++i)
+        for (int j = 0; j < nh; ++j) G[i * nh + j] = s.gram[hist[i] * D + hist[j]];
+      if (!diis_coefficients(G, nh, c)) {
+        s.order.assign(1, s.cur);
+        c.assign(1, 1.0);
+      }
+      const int nh2 = (int)s.order.size();
+      // next slot: one not in the history, else the oldest
+      int next = -1;
+      for (int q = 0; q < D && next < 0; ++q)
+        if (std::find(s.order.begin(), s.order.end(), q) == s.order.end()) next = q;
+      if (next < 0) next = s.order.front();
+      dd[b].active = 1;
+      dd[b].nh = nh2;
+      for (int j = 0; j < nh2; ++j) {
+        dd[b].rh[j] = s.rh.p + (int64_t)s.order[j] * mP;
+        dd[b].th[j] = s.th.p + (int64_t)s.order[j] * mP;
+        dd[b].coef[j] = c[j];
+      }
+      dd[b].lam = cf.lam;
+      dd[b].tout = s.th.p + (int64_t)next * mP;
+      if (next == s.order.front() && nh2 == D) s.order.erase(s.order.begin());
+      s.cur = next;
+      any_update = true;
+    }
+    if (!fp32 && cf.precision == RISM3D_MIXED) polish_left--;
+    if (any_update) {
+      HIPCHK(hipMemcpyAsync(pb.d_dd.p, dd.data(), B * sizeof(DiisDesc), hipMemcpyHostToDevice, pb.s));
+      hipLaunchKernelGGL(diis_update, gchunk, dim3(NT), 0, pb.s, pb.d_dd.p, mP);
+    }
+  }
+}
+
+struct Inputs {
+  const rism3d_problem* pd;
+  int m;
+  const rism3d_site* solvent;
+  int B;
+  const int* n_sites;
+  const rism3d_site* const* solutes;
+  const double* chi_k;
+  rism3d_options opt;
+  const double* t_eval = nullptr;  // residual_only: t in
+  double* r_out = nullptr;         // residual_only: F(t) out
+};
+
+int validate(const Inputs& in) {
+  const rism3d_problem* pd = in.pd;
+  if (!pd || !in.solvent || in.m < 1 || in.m > MAX_SITES || in.B < 1) return RISM3D_BAD_ARG;
+  for (int d = 0; d < 3; ++d) {
+    if (!supported_n(pd->N[d])) return RISM3D_UNSUPPORTED;
+    if (!(pd->L[d] > 0)) return RISM3D_BAD_ARG;
+  }
+  if (!(pd->rmax > 0) || pd->nrad < 4 || !(pd->beta > 0) || !(pd->rho > 0)) return RISM3D_BAD_ARG;
+  if (pd->closure < 0 || pd->closure > RISM3D_CLOSURE_PSE7) return RISM3D_BAD_ARG;
+  for (int b = 0; b < in.B; ++b)
+    if (in.n_sites[b] < 1 || !in.solutes[b]) return RISM3D_BAD_ARG;
+  // the k table must reach the grid corner: interp asserts otherwise
+  // (bgy3d-vec.c:31-32)
+  double k2 = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double kd = (pd->N[d] / 2) * 2 * M_PI / pd->L[d];
+    k2 += kd * kd;
+  }
+  const double dk = M_PI / pd->rmax;
+  const int imax = (int)(std::sqrt(k2) / dk - 0.5 + 0.5);
+  if (imax > pd->nrad - 1) return RISM3D_BAD_ARG;
+  if (in.opt.diis_depth < 1 || in.opt.diis_depth > MAX_DIIS) return RISM3D_BAD_ARG;
+  // z-pass LDS: m lines of N2 + 1 complex per kx column, at least 2 columns
+  {
+    const size_t lds64 = (size_t)in.m * zstar_lines<double>(pd->N[2], in.m) * (pd->N[2] + 1) * 16;
+    if (lds64 > 64 * 1024) return RISM3D_UNSUPPORTED;
+  }
+  return RISM3D_OK;
+}
+
+std::mutex g_mutex;  // one solve per process at a time (shared streams)
+
+int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
+        std::vector<double>* t_final) {
+  int v = validate(in);
+  if (v != RISM3D_OK) {
+    for (int b = 0; b < in.B; ++b) res[b].status = v;
+    return v;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= in.opt.device) {
+    for (int b = 0; b < in.B; ++b) res[b].status = RISM3D_NO_DEVICE;
+    return RISM3D_NO_DEVICE;
+  }
+  std::lock_guard<std::mutex> lock(g_mutex);
+  HIPCHK(hipSetDevice(in.opt.device));
+  const rism3d_problem& pd = *in.pd;
+  Problem pb;
+  pb.B = in.B;
+  pb.D = in.opt.solver == RISM3D_SOLVER_PICARD ? 2 : in.opt.diis_depth;
+  pb.mixed = in.opt.precision == RISM3D_MIXED;
+  for (int d = 0; d < 3; ++d) { pb.N[d] = pd.N[d]; pb.L[d] = pd.L[d]; pb.h[d] = pd.L[d] / pd.N[d]; }
+  GridArgs& g = pb.g;
+  std::memset(&g, 0, sizeof g);
+  g.N0 = pd.N[0]; g.N1 = pd.N[1]; g.N2 = pd.N[2]; g.NH = pd.N[0] / 2 + 1;
+  g.m = in.m;
+  g.nrad = pd.nrad;
+  g.P = (int64_t)g.N0 * g.N1 * g.N2;
+  g.K = (int64_t)g.NH * g.N1 * g.N2;
+  g.dkx = 2 * M_PI / pd.L[0]; g.dky = 2 * M_PI / pd.L[1]; g.dkz = 2 * M_PI / pd.L[2];
+  g.dk_tab = M_PI / pd.rmax;
+  const double h3 = pb.h[0] * pb.h[1] * pb.h[2];
+  const double L3 = pd.L[0] * pd.L[1] * pd.L[2];
+  g.h3 = h3;
+  g.inv_L3 = 1.0 / L3;
+  g.a_scale = h3 / L3;
+  g.tau_scale = pd.beta * EPSILON0INV / L3;
+  g.beta = pd.beta;
+  g.renorm = in.opt.renorm ? 1 : 0;
+  g.closure = pd.closure;
+  g.pse_order = pd.closure == RISM3D_CLOSURE_KH ? 1 : (pd.closure >= RISM3D_CLOSURE_PSE0 ? pd.closure - RISM3D_CLOSURE_PSE0 : 0);
+  for (int i = 0; i < in.m; ++i) g.q[i] = in.solvent[i].charge;
+  pb.mP = (int64_t)in.m * g.P;
+
+  HIPCHK(hipStreamCreateWithFlags(&pb.s, hipStreamNonBlocking));
+  struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{pb.s};
+  make_twiddles(pb.tw, pb.N, pb.s);
+
+  // Solvent kernel: chi - delta and the renormalisation tables
+  // (solvent_kernel_rism, hnc3d.c:1499-1566; rism_solvent_renorm,
+  // rism.f90:141-208).  chi_k is C layout [m][m][nrad].
+  const int m = in.m, nrad = pd.nrad;
+  std::vector<double> chi_tab((size_t)m * (m + 1) / 2 * nrad), tau_tab((size_t)m * nrad);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j <= i; ++j)
+      for (int k = 0; k < nrad; ++k)
+        chi_tab[((size_t)i * (i + 1) / 2 + j) * nrad + k] =
+            in.chi_k[((size_t)i * m + j) * nrad + k] - (i == j ? 1.0 : 0.0);
+  for (int i = 0; i < m; ++i)
+    for (int k = 0; k < nrad; ++k) {
+      const double kk = (2 * k + 1) * g.dk_tab / 2;
+      const double v = 4 * M_PI * std::exp(-kk * kk / (4 * G_COULOMB * G_COULOMB)) / (kk * kk);
+      double x = 0.0;
+      for (int j = 0; j < m; ++j) x += in.chi_k[((size_t)j * m + i) * nrad + k] * in.solvent[j].charge;
+      tau_tab[(size_t)i * nrad + k] = x * v;
+    }
+  pb.chi_tab = DevBuf<double>(chi_tab.size());
+  pb.tau_tab = DevBuf<double>(tau_tab.size());
+  HIPCHK(hipMemcpyAsync(pb.chi_tab.p, chi_tab.data(), chi_tab.size() * 8, hipMemcpyHostToDevice, pb.s));
+  HIPCHK(hipMemcpyAsync(pb.tau_tab.p, tau_tab.data(), tau_tab.size() * 8, hipMemcpyHostToDevice, pb.s));
+
+  // per-solute buffers
+  const int B = in.B;
+  pb.st.resize(B);
+  pb.nparts = resid_grid_x(g) * m;
+  const int64_t mK = (int64_t)m * g.K;
+  std::vector<SitesDesc> sites(B);
+  std::vector<double*> bv64p(B), tp(B), gp(B);
+  std::vector<float*> bv32p(B);
+  std::vector<cpx<double>*> fs64p(B);
+  std::vector<cpx<float>*> fs32p(B);
+  std::vector<std::vector<double>> site_host(B);
+  for (int b = 0; b < B; ++b) {
+    auto& s = pb.st[b];
+    s.th = DevBuf<double>((size_t)pb.D * pb.mP);
+    s.rh = DevBuf<double>((size_t)pb.D * pb.mP);
+    s.bv64 = DevBuf<double>(pb.mP);
+    if (pb.mixed) s.bv32 = DevBuf<float>(pb.mP);
+    s.W = DevBuf<cpx<double>>(mK);
+    s.fs64 = DevBuf<cpx<double>>(g.K);
+    if (pb.mixed) s.fs32 = DevBuf<cpx<float>>(g.K);
+    s.part = DevBuf<double>((size_t)pb.nparts * 2);
+    s.n_sites = in.n_sites[b];
+    auto& sh = site_host[b];
+    sh.resize((size_t)s.n_sites * 6);
+    for (int a = 0; a < s.n_sites; ++a) {
+      const rism3d_site& st = in.solutes[b][a];
+      sh[6 * a + 0] = st.x[0]; sh[6 * a + 1] = st.x[1]; sh[6 * a + 2] = st.x[2];
+      sh[6 * a + 3] = st.sigma; sh[6 * a + 4] = st.epsilon; sh[6 * a + 5] = st.charge;
+    }
+    s.sites = DevBuf<double>(sh.size());
+    HIPCHK(hipMemcpyAsync(s.sites.p, sh.data(), sh.size() * 8, hipMemcpyHostToDevice, pb.s));
+    sites[b].xyz_s_e_q = s.sites.p;
+    sites[b].n = s.n_sites;
+    bv64p[b] = s.bv64.p;
+    bv32p[b] = pb.mixed ? s.bv32.p : nullptr;
+    fs64p[b] = s.fs64.p;
+    fs32p[b] = pb.mixed ? s.fs32.p : nullptr;
+    // initial iterate: zero (hnc3d.c:1845) or the restart token
+    if (t_init && b == 0)
+      HIPCHK(hipMemcpyAsync(s.th.p, t_init->data(), pb.mP * 8, hipMemcpyHostToDevice, pb.s));
+    else
+      HIPCHK(hipMemsetAsync(s.th.p, 0, pb.mP * 8, pb.s));
+  }
+  pb.d_sd = DevBuf<SoluteDesc>(B);
+  pb.d_dd = DevBuf<DiisDesc>(B);
+  pb.d_sites = DevBuf<SitesDesc>(B);
+  pb.d_out = DevBuf<double>(2 * B);
+  pb.nchunks = (int)std::min<int64_t>(1024, (pb.mP + NT * 8 - 1) / (NT * 8));
+  pb.d_dotpart = DevBuf<double>((size_t)B * pb.nchunks * MAX_DIIS);
+  pb.d_dots = DevBuf<double>((size_t)B * MAX_DIIS);
+  HIPCHK(hipMemcpyAsync(pb.d_sites.p, sites.data(), B * sizeof(SitesDesc), hipMemcpyHostToDevice, pb.s));
+
+  // setup: beta v (field0) and F(k) s(k)
+  {
+    std::vector<double> sse(3 * m);
+    for (int i = 0; i < m; ++i) {
+      sse[3 * i] = in.solvent[i].sigma;
+      sse[3 * i + 1] = in.solvent[i].epsilon;
+      sse[3 * i + 2] = in.solvent[i].charge;
+    }
+    DevBuf<double> d_sse(sse.size());
+    DevBuf<double*> d_bv64(B);
+    DevBuf<float*> d_bv32(B);
+    DevBuf<cpx<double>*> d_fs64(B);
+    DevBuf<cpx<float>*> d_fs32(B);
+    HIPCHK(hipMemcpyAsync(d_sse.p, sse.data(), sse.size() * 8, hipMemcpyHostToDevice, pb.s));
+    HIPCHK(hipMemcpyAsync(d_bv64.p, bv64p.data(), B * sizeof(double*), hipMemcpyHostToDevice, pb.s));
+    HIPCHK(hipMemcpyAsync(d_bv32.p, bv32p.data(), B * sizeof(float*), hipMemcpyHostToDevice, pb.s));
+    HIPCHK(hipMemcpyAsync(d_fs64.p, fs64p.data(), B * sizeof(void*), hipMemcpyHostToDevice, pb.s));
+    HIPCHK(hipMemcpyAsync(d_fs32.p, fs32p.data(), B * sizeof(void*), hipMemcpyHostToDevice, pb.s));
+    const int cp = (int)std::min<int64_t>(4096, (g.P + NT - 1) / NT);
+    const int ck = (int)std::min<int64_t>(4096, (g.K + NT - 1) / NT);
+    hipLaunchKernelGGL(field_bv, dim3(cp, B), dim3(NT), 0, pb.s, pb.d_sites.p, g, d_sse.p,
+                       pd.L[0], pd.L[1], pd.L[2], pb.h[0], pb.h[1], pb.h[2], d_bv64.p,
+                       pb.mixed ? d_bv32.p : (float* const*)nullptr);
+    hipLaunchKernelGGL(form_factor, dim3(ck, B), dim3(NT), 0, pb.s, pb.d_sites.p, g, d_fs64.p,
+                       pb.mixed ? d_fs32.p : (cpx<float>* const*)nullptr);
+    HIPCHK(hipStreamSynchronize(pb.s));
+    HIPCHK(hipGetLastError());
+  }
+
+  if (in.t_eval) {
+    // one residual evaluation F(t) = T(t) - t (iterate_t1, hnc3d.c:1184)
+    auto& s0 = pb.st[0];
+    HIPCHK(hipMemcpyAsync(s0.th.p, in.t_eval, pb.mP * 8, hipMemcpyHostToDevice, pb.s));
+    std::vector<SoluteDesc> sd(1);
+    sd[0].t = s0.th.p;
+    sd[0].r = s0.rh.p;
+    sd[0].bv = pb.mixed ? (const void*)s0.bv32.p : (const void*)s0.bv64.p;
+    sd[0].W = s0.W.p;
+    sd[0].Fs = pb.mixed ? (const void*)s0.fs32.p : (const void*)s0.fs64.p;
+    sd[0].part = s0.part.p;
+    sd[0].active = 1;
+    evaluate(pb, pb.mixed != 0, sd);
+    HIPCHK(hipMemcpyAsync(in.r_out, s0.rh.p, pb.mP * 8, hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+    HIPCHK(hipGetLastError());
+    res[0].status = RISM3D_OK;
+    return RISM3D_OK;
+  }
+
+  Config cf;
+  cf.solver = in.opt.solver;
+  cf.precision = in.opt.precision;
+  cf.polish = in.opt.polish_iters;
+  cf.depth = pb.D;
+  cf.verbose = in.opt.verbose;
+  cf.max_iter = pd.max_iter > 0 ? pd.max_iter : 1000;
+  cf.lam = in.opt.solver == RISM3D_SOLVER_PICARD ? pd.lambda : in.opt.diis_step;
+  cf.tol = pd.norm_tol;
+  cf.mixed_tol = in.opt.mixed_tol;
+  iterate_batch(pb, cf);
+
+  // observables in FP64 from the converged t (hnc3d.c:2157-2268)
+  {
+    std::vector<const double*> tt(B);
+    std::vector<double*> gg(B, nullptr);
+    std::vector<DevBuf<double>> gbuf(B);
+    bool want_g = false;
+    for (int b = 0; b < B; ++b) {
+      tt[b] = pb.st[b].th.p + (int64_t)pb.st[b].cur * pb.mP;
+      if (res[b].g) {
+        gbuf[b] = DevBuf<double>(pb.mP);
+        gg[b] = gbuf[b].p;
+        want_g = true;
+      }
+    }
+    DevBuf<const double*> d_tt(B);
+    DevBuf<double*> d_gg(B);
+    DevBuf<const double*> d_bv(B);
+    std::vector<const double*> bvc(bv64p.begin(), bv64p.end());
+    HIPCHK(hipMemcpyAsync(d_tt.p, tt.data(), B * sizeof(void*), hipMemcpyHostToDevice, pb.s));
+    HIPCHK(hipMemcpyAsync(d_gg.p, gg.data(), B * sizeof(void*), hipMemcpyHostToDevice, pb.s));
+    HIPCHK(hipMemcpyAsync(d_bv.p, bvc.data(), B * sizeof(void*), hipMemcpyHostToDevice, pb.s));
+    const int cp = (int)std::min<int64_t>(2048, (g.P + NT - 1) / NT);
+    DevBuf<double> d_mu((size_t)B * cp * 5);
+    hipLaunchKernelGGL(mu_density, dim3(cp, B), dim3(NT), 0, pb.s, pb.d_sites.p, g, d_tt.p, d_bv.p,
+                       pd.L[0], pd.L[1], pd.L[2], pb.h[0], pb.h[1], pb.h[2],
+                       want_g ? d_gg.p : (double* const*)nullptr, d_mu.p);
+    std::vector<double> mu((size_t)B * cp * 5);
+    HIPCHK(hipMemcpyAsync(mu.data(), d_mu.p, mu.size() * 8, hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+    HIPCHK(hipGetLastError());
+    for (int b = 0; b < B; ++b) {
+      double acc[5] = {0, 0, 0, 0, 0}, comp[5] = {0, 0, 0, 0, 0};
+      for (int c = 0; c < cp; ++c)
+        for (int q = 0; q < 5; ++q) {  // Neumaier, fixed order
+          const double v = mu[((size_t)b * cp + c) * 5 + q];
+          const double s = acc[q] + v;
+          comp[q] += (std::fabs(acc[q]) >= std::fabs(v)) ? (acc[q] - s) + v : (v - s) + acc[q];
+          acc[q] = s;
+        }
+      for (int q = 0; q < 5; ++q) acc[q] += comp[q];
+      const double f = pd.rho * h3 / pd.beta;
+      auto& s = pb.st[b];
+      res[b].mu = f * acc[0];
+      res[b].mu_hnc = f * acc[1];
+      res[b].mu_kh = f * acc[2];
+      res[b].mu_gf = f * acc[3];
+      res[b].pmv_factor = 1.0 - acc[4] * pd.rho * h3;
+      res[b].residual = s.resid;
+      res[b].iterations = s.iters;
+      res[b].iterations_fp32 = s.iters32;
+      res[b].status = s.status;
+      if (res[b].g)
+        HIPCHK(hipMemcpy(res[b].g, gbuf[b].p, pb.mP * 8, hipMemcpyDeviceToHost));
+      if (res[b].t)
+        HIPCHK(hipMemcpy(res[b].t, tt[b], pb.mP * 8, hipMemcpyDeviceToHost));
+      if (t_final && b == 0) {
+        t_final->resize(pb.mP);
+        HIPCHK(hipMemcpy(t_final->data(), tt[b], pb.mP * 8, hipMemcpyDeviceToHost));
+      }
+    }
+  }
+  int worst = RISM3D_OK;
+  for (int b = 0; b < B; ++b) worst = std::max(worst, res[b].status);
+  return worst;
+}
+
+}  // namespace
+}  // namespace rism
+
+struct rism3d_restart {
+  int N[3];
+  int m;
+  std::vector<double> t;
+};
+
+extern "C" {
+
+void rism3d_options_default(rism3d_options* opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof *opt);
+  opt->renorm = 1;
+  opt->solver = RISM3D_SOLVER_MDIIS;
+  opt->precision = RISM3D_FP64;
+  opt->polish_iters = 3;
+  opt->mixed_tol = 1e-5;
+  opt->diis_depth = 8;
+  opt->diis_step = 0.5;
+  opt->verbose = 0;
+  opt->device = 0;
+  opt->dielectric = 0.0;
+}
+
+int rism3d_solvent_chi_internal(const rism3d_problem* pd, int m, const rism3d_site* solvent,
+                                double dielectric, double* chi_out);
+
+static int solve_impl(const rism3d_problem* pd, int m, const rism3d_site* solvent, int B,
+                      const int* n_sites, const rism3d_site* const* solutes, const double* chi_k,
+                      const rism3d_options* opt, rism3d_result* res, rism3d_restart** restart) {
+  if (!pd || !res || !n_sites || !solutes) return RISM3D_BAD_ARG;
+  rism::Inputs in;
+  in.pd = pd;
+  in.m = m;
+  in.solvent = solvent;
+  in.B = B;
+  in.n_sites = n_sites;
+  in.solutes = solutes;
+  if (opt) in.opt = *opt; else rism3d_options_default(&in.opt);
+  std::vector<double> chi_own;
+  rism3d_problem pd_up;
+  if (!chi_k) {
+    // chi from 1D DRISM on the upscaled radial grid (hnc3d.c:772-783,
+    // 1471-1495): rmax = 4 max(L)/2, nrad = 16 max(N).
+    pd_up = *pd;
+    pd_up.rmax = 4 * std::max(std::max(pd->L[0], pd->L[1]), pd->L[2]) / 2;
+    pd_up.nrad = 16 * std::max(std::max(pd->N[0], pd->N[1]), pd->N[2]);
+    chi_own.resize((size_t)m * m * pd_up.nrad);
+    int st = rism3d_solvent_chi_internal(&pd_up, m, solvent, in.opt.dielectric, chi_own.data());
+    if (st != RISM3D_OK) {
+      for (int b = 0; b < B; ++b) res[b].status = st;
+      return st;
+    }
+    in.pd = &pd_up;
+    chi_k = chi_own.data();
+  }
+  in.chi_k = chi_k;
+  std::vector<double> t_init, t_final;
+  std::vector<double>* ti = nullptr;
+  if (restart && *restart) {
+    rism3d_restart* r = *restart;
+    if (r->m == m && r->N[0] == pd->N[0] && r->N[1] == pd->N[1] && r->N[2] == pd->N[2]) {
+      t_init = std::move(r->t);
+      ti = &t_init;
+    }
+    delete r;
+    *restart = nullptr;
+  }
+  try {
+    int st = rism::run(in, res, ti, restart ? &t_final : nullptr);
+    if (restart && !t_final.empty()) {
+      rism3d_restart* r = new rism3d_restart;
+      for (int d = 0; d < 3; ++d) r->N[d] = pd->N[d];
+      r->m = m;
+      r->t = std::move(t_final);
+      *restart = r;
+    }
+    return st;
+  } catch (const rism::HipFailure& e) {
+    const int st = (e.err == hipErrorOutOfMemory) ? RISM3D_OOM : RISM3D_NO_DEVICE;
+    for (int b = 0; b < B; ++b) res[b].status = st;
+    return st;
+  }
+}
+
+int rism3d_solute_solve(const rism3d_problem* pd, int m, const rism3d_site* solvent, int n,
+                        const rism3d_site* solute, const double* chi_k, const rism3d_options* opt,
+                        rism3d_result* res, rism3d_restart** restart) {
+  const rism3d_site* sol[1] = {solute};
+  int ns[1] = {n};
+  return solve_impl(pd, m, solvent, 1, ns, sol, chi_k, opt, res, restart);
+}
+
+int rism3d_batch_solve(const rism3d_problem* pd, int m, const rism3d_site* solvent, int n_solutes,
+                       const int* n_sites, const rism3d_site* const* solutes, const double* chi_k,
+                       const rism3d_options* opt, rism3d_result* res) {
+  return solve_impl(pd, m, solvent, n_solutes, n_sites, solutes, chi_k, opt, res, nullptr);
+}
+
+void rism3d_restart_destroy(rism3d_restart* restart) { delete restart; }
+
+int rism3d_residual(const rism3d_problem* pd, int m, const rism3d_site* solvent, int n,
+                    const rism3d_site* solute, const double* chi_k, const rism3d_options* opt,
+                    const double* t, double* r) {
+  if (!pd || !chi_k || !t || !r || !solute) return RISM3D_BAD_ARG;
+  rism::Inputs in;
+  in.pd = pd;
+  in.m = m;
+  in.solvent = solvent;
+  in.B = 1;
+  const rism3d_site* sol[1] = {solute};
+  int ns[1] = {n};
+  in.n_sites = ns;
+  in.solutes = sol;
+  in.chi_k = chi_k;
+  if (opt) in.opt = *opt; else rism3d_options_default(&in.opt);
+  in.t_eval = t;
+  in.r_out = r;
+  rism3d_result res;
+  std::memset(&res, 0, sizeof res);
+  try {
+    return rism::run(in, &res, nullptr, nullptr);
+  } catch (const rism::HipFailure& e) {
+    return (e.err == hipErrorOutOfMemory) ? RISM3D_OOM : RISM3D_NO_DEVICE;
+  }
+}
+
+int rism3d_solvent_chi(const rism3d_problem* pd, int m, const rism3d_site* solvent,
+                       double dielectric, double* chi_out) {
+  return rism3d_solvent_chi_internal(pd, m, solvent, dielectric, chi_out);
+}
+
+const char* rism3d_version(void) { return "rism3d-mi355x 0.1 (gfx950)"; }
+
+}  // extern "C"

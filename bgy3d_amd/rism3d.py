@@ -1,0 +1,315 @@
+"""Python host layer over ``librism3d.so`` (the C ABI of include/rism3d.h).
+
+Mirrors the reference's Guile-facing interface of the 3D-RISM solute path:
+
+* ``problem_data(settings)``  -- bgy3d-guile.c:243-385 (defaults, --L is the
+  half box edge, h = L/N, rmax = max(L)/2 and nrad = max(N) unless given)
+* ``hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None)``
+  -- guile/bgy3d.scm:386-388 -> bgy3d-guile.c:1091-1209 run_solute ->
+  hnc3d_solute_solve; returns a dict with the same keys as the reference's
+  result alist ('free-energy, 'partial-molar-volume-factor, GUV, RESTART)
+* ``rism_solvent(solvent, settings)`` -- guile/bgy3d.scm:358 ->
+  bgy3d-guile.c:1233-1286; returns {'susceptibility': chi[m][m][nrad]}
+
+The compute path is the gfx950 HIP library only.  There is no CPU fallback:
+if the library is missing this module raises, and on a machine without a
+GPU the C ABI returns RISM3D_NO_DEVICE, which raises here.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .solutes import find_molecule
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(_HERE, "librism3d.so")
+
+CLOSURES = {"HNC": 0, "KH": 1, "PY": 2, **{f"PSE{n}": 3 + n for n in range(8)}}
+
+OK, NONCONVERGED, NAN, BAD_ARG, NO_DEVICE, OOM, UNSUPPORTED = range(7)
+STATUS_NAMES = ["OK", "NONCONVERGED", "NAN", "BAD_ARG", "NO_DEVICE", "OOM", "UNSUPPORTED"]
+
+SOLVER_MDIIS, SOLVER_PICARD = 0, 1
+FP64, MIXED = 0, 1
+
+
+class RismError(RuntimeError):
+    def __init__(self, status, what="rism3d"):
+        self.status = status
+        name = STATUS_NAMES[status] if 0 <= status < len(STATUS_NAMES) else str(status)
+        super().__init__(f"{what}: status {name}")
+
+
+class Problem(ctypes.Structure):
+    """ProblemData, bgy3d.h:228-256."""
+    _fields_ = [("N", ctypes.c_int * 3), ("L", ctypes.c_double * 3),
+                ("h", ctypes.c_double * 3), ("rmax", ctypes.c_double),
+                ("nrad", ctypes.c_int), ("beta", ctypes.c_double),
+                ("rho", ctypes.c_double), ("lambda_", ctypes.c_double),
+                ("damp", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("norm_tol", ctypes.c_double), ("closure", ctypes.c_int)]
+
+
+class Site(ctypes.Structure):
+    """Site, bgy3d-solutes.h:9-16."""
+    _fields_ = [("name", ctypes.c_char * 5), ("x", ctypes.c_double * 3),
+                ("sigma", ctypes.c_double), ("epsilon", ctypes.c_double),
+                ("charge", ctypes.c_double), ("user", ctypes.c_void_p)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("renorm", ctypes.c_int), ("solver", ctypes.c_int),
+                ("precision", ctypes.c_int), ("polish_iters", ctypes.c_int),
+                ("mixed_tol", ctypes.c_double), ("diis_depth", ctypes.c_int),
+                ("diis_step", ctypes.c_double), ("verbose", ctypes.c_int),
+                ("device", ctypes.c_int), ("dielectric", ctypes.c_double)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("mu", ctypes.c_double), ("mu_hnc", ctypes.c_double),
+                ("mu_kh", ctypes.c_double), ("mu_gf", ctypes.c_double),
+                ("pmv_factor", ctypes.c_double), ("residual", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("iterations_fp32", ctypes.c_int),
+                ("status", ctypes.c_int),
+                ("g", ctypes.POINTER(ctypes.c_double)),
+                ("t", ctypes.POINTER(ctypes.c_double))]
+
+
+EXPORTS = ("rism3d_options_default", "rism3d_solute_solve", "rism3d_batch_solve",
+           "rism3d_residual", "rism3d_restart_destroy", "rism3d_solvent_chi",
+           "rism3d_version")
+
+_lib = None
+
+
+def load():
+    """Load the in-tree gfx950 library; raise if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            raise RuntimeError(f"{LIBRARY} is missing: run `python -m bgy3d_amd.build`")
+        lib = ctypes.CDLL(LIBRARY)
+        P = ctypes.POINTER
+        dp = P(ctypes.c_double)
+        lib.rism3d_options_default.argtypes = [P(Options)]
+        lib.rism3d_options_default.restype = None
+        lib.rism3d_solute_solve.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_int,
+                                            P(Site), dp, P(Options), P(Result),
+                                            P(ctypes.c_void_p)]
+        lib.rism3d_solute_solve.restype = ctypes.c_int
+        lib.rism3d_batch_solve.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_int,
+                                           P(ctypes.c_int), P(P(Site)), dp, P(Options),
+                                           P(Result)]
+        lib.rism3d_batch_solve.restype = ctypes.c_int
+        lib.rism3d_residual.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_int,
+                                        P(Site), dp, P(Options), dp, dp]
+        lib.rism3d_residual.restype = ctypes.c_int
+        lib.rism3d_restart_destroy.argtypes = [ctypes.c_void_p]
+        lib.rism3d_restart_destroy.restype = None
+        lib.rism3d_solvent_chi.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_double, dp]
+        lib.rism3d_solvent_chi.restype = ctypes.c_int
+        lib.rism3d_version.argtypes = []
+        lib.rism3d_version.restype = ctypes.c_char_p
+        _lib = lib
+    return _lib
+
+
+# ------------------------------------------------------------ settings
+DEFAULTS = {
+    # bgy3d-guile.c:249-270 and guile/bgy3d.scm:323-340
+    "beta": 1.6889, "rho": 0.033427745, "lambda": 0.02, "damp-start": 1.0,
+    "max-iter": 1500, "norm-tol": 1.0e-7, "L": 10.0, "N": 64, "closure": "HNC",
+}
+
+
+def problem_data(settings):
+    """bgy3d-guile.c:243-385: settings dict -> ProblemData."""
+    s = dict(DEFAULTS)
+    s.update(settings or {})
+    pd = Problem()
+    n = int(s["N"])
+    length = float(s["L"])
+    L = [2.0 * length] * 3
+    for i, k in enumerate(("sx", "sy", "sz")):
+        if k in s:
+            L[i] = float(s[k])
+    N = [n] * 3
+    for i, k in enumerate(("nx", "ny", "nz")):
+        if k in s:
+            N[i] = int(s[k])
+    for i in range(3):
+        pd.N[i] = N[i]
+        pd.L[i] = L[i]
+        pd.h[i] = L[i] / N[i]
+    pd.rmax = float(s.get("rmax", max(L) / 2))
+    pd.nrad = int(s.get("nrad", max(N)))
+    pd.beta = float(s["beta"])
+    pd.rho = float(s["rho"])
+    pd.lambda_ = float(s["lambda"])
+    pd.damp = float(s["damp-start"])
+    pd.max_iter = int(s["max-iter"])
+    pd.norm_tol = float(s["norm-tol"])
+    clo = s["closure"]
+    pd.closure = CLOSURES[clo] if isinstance(clo, str) else int(clo)
+    return pd
+
+
+def make_sites(sites):
+    """(name, (x,y,z), sigma, epsilon, charge) tuples -> Site array."""
+    if isinstance(sites, str):
+        sites = find_molecule(sites)
+    arr = (Site * len(sites))()
+    for i, (name, x, sig, eps, q) in enumerate(sites):
+        arr[i].name = name.encode()[:4]
+        for d in range(3):
+            arr[i].x[d] = float(x[d])
+        arr[i].sigma = float(sig)
+        arr[i].epsilon = float(eps)
+        arr[i].charge = float(q)
+    return arr
+
+
+def make_options(settings=None, **kw):
+    s = dict(settings or {})
+    s.update(kw)
+    opt = Options()
+    load().rism3d_options_default(ctypes.byref(opt))
+    if s.get("no-renorm"):
+        opt.renorm = 0
+    if "snes-solver" in s:
+        opt.solver = {"mdiis": SOLVER_MDIIS, "picard": SOLVER_PICARD}[s["snes-solver"]]
+    for key, field in (("precision", "precision"), ("polish-iters", "polish_iters"),
+                       ("mixed-tol", "mixed_tol"), ("diis-depth", "diis_depth"),
+                       ("diis-step", "diis_step"), ("verbose", "verbose"),
+                       ("device", "device"), ("dielectric", "dielectric")):
+        if key in s:
+            v = s[key]
+            if key == "precision" and isinstance(v, str):
+                v = {"fp64": FP64, "mixed": MIXED}[v]
+            setattr(opt, field, type(getattr(opt, field))(v))
+    return opt
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=False):
+    """Solve the 3D-RISM solute problem on the GPU.
+
+    Returns a dict like the reference alist: 'free-energy',
+    'partial-molar-volume-factor', 'GUV' (g, [m][N2][N1][N0]), plus
+    'free-energy-hnc'/'-kh'/'-gf', 'iterations', 'residual', 'RESTART'
+    (opaque token, pass back as ``restart=`` and free with
+    ``restart_destroy``) if ``restart`` is not None."""
+    lib = load()
+    pd = problem_data(settings)
+    opt = make_options(settings)
+    sv = make_sites(solvent)
+    su = make_sites(solute)
+    m = len(sv)
+    shape = (m, pd.N[2], pd.N[1], pd.N[0])
+    g = np.empty(shape)
+    res = Result()
+    res.g = _dptr(g)
+    t = None
+    if want_t:
+        t = np.empty(shape)
+        res.t = _dptr(t)
+    chi_p = None
+    if chi is not None:
+        chi = np.ascontiguousarray(chi, dtype=np.float64)
+        if chi.shape != (m, m, pd.nrad):
+            raise ValueError(f"chi must be (m, m, nrad) = {(m, m, pd.nrad)}, got {chi.shape}")
+        chi_p = _dptr(chi)
+    tok = None
+    if restart is not None:
+        tok = ctypes.c_void_p(restart if isinstance(restart, int) else restart)
+    st = lib.rism3d_solute_solve(ctypes.byref(pd), m, sv, len(su), su, chi_p, ctypes.byref(opt),
+                                 ctypes.byref(res), ctypes.byref(tok) if tok is not None else None)
+    if st not in (OK, NONCONVERGED):
+        raise RismError(st, "rism3d_solute_solve")
+    out = {
+        "free-energy": res.mu,
+        "partial-molar-volume-factor": res.pmv_factor,
+        "free-energy-hnc": res.mu_hnc, "free-energy-kh": res.mu_kh,
+        "free-energy-gf": res.mu_gf,
+        "GUV": g, "iterations": res.iterations, "iterations-fp32": res.iterations_fp32,
+        "residual": res.residual, "status": STATUS_NAMES[st],
+    }
+    if t is not None:
+        out["t"] = t
+    if tok is not None:
+        out["RESTART"] = tok.value
+    return out
+
+
+def restart_destroy(token):
+    if token:
+        load().rism3d_restart_destroy(ctypes.c_void_p(token))
+
+
+def batch_solve(solutes, solvent, settings, chi, want_g=False):
+    """Many solutes, one grid and solvent: rism3d_batch_solve."""
+    lib = load()
+    pd = problem_data(settings)
+    opt = make_options(settings)
+    sv = make_sites(solvent)
+    m = len(sv)
+    arrs = [make_sites(s) for s in solutes]
+    B = len(arrs)
+    ptrs = (ctypes.POINTER(Site) * B)(*[ctypes.cast(a, ctypes.POINTER(Site)) for a in arrs])
+    ns = (ctypes.c_int * B)(*[len(a) for a in arrs])
+    res = (Result * B)()
+    gs = []
+    if want_g:
+        for b in range(B):
+            g = np.empty((m, pd.N[2], pd.N[1], pd.N[0]))
+            gs.append(g)
+            res[b].g = _dptr(g)
+    chi = np.ascontiguousarray(chi, dtype=np.float64)
+    st = lib.rism3d_batch_solve(ctypes.byref(pd), m, sv, B, ns, ptrs, _dptr(chi),
+                                ctypes.byref(opt), res)
+    if st not in (OK, NONCONVERGED):
+        raise RismError(st, "rism3d_batch_solve")
+    out = []
+    for b in range(B):
+        d = {"free-energy": res[b].mu, "partial-molar-volume-factor": res[b].pmv_factor,
+             "iterations": res[b].iterations, "iterations-fp32": res[b].iterations_fp32,
+             "residual": res[b].residual, "status": STATUS_NAMES[res[b].status]}
+        if want_g:
+            d["GUV"] = gs[b]
+        out.append(d)
+    return out
+
+
+def residual(solute, solvent, settings, chi, t):
+    """One evaluation of F(t) = (chi - 1) * c(t) - t (iterate_t1)."""
+    lib = load()
+    pd = problem_data(settings)
+    opt = make_options(settings)
+    sv = make_sites(solvent)
+    su = make_sites(solute)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    r = np.empty_like(t)
+    chi = np.ascontiguousarray(chi, dtype=np.float64)
+    st = lib.rism3d_residual(ctypes.byref(pd), len(sv), sv, len(su), su, _dptr(chi),
+                             ctypes.byref(opt), _dptr(t), _dptr(r))
+    if st != OK:
+        raise RismError(st, "rism3d_residual")
+    return r
+
+
+def rism_solvent(solvent, settings):
+    """1D DRISM susceptibility (rism_solvent, rism.f90:57-102)."""
+    lib = load()
+    pd = problem_data(settings)
+    sv = make_sites(solvent)
+    m = len(sv)
+    chi = np.empty((m, m, pd.nrad))
+    eps = float((settings or {}).get("dielectric", 0.0))
+    st = lib.rism3d_solvent_chi(ctypes.byref(pd), m, sv, eps, _dptr(chi))
+    if st != OK:
+        raise RismError(st, "rism3d_solvent_chi")
+    return {"susceptibility": chi}

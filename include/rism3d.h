@@ -1,0 +1,188 @@
+/*
+ * rism3d.h -- C ABI of the MI355X-native 3D-RISM solute-in-solvent solver.
+ *
+ * This is the drop-in boundary for the reference's 3D-RISM solute path:
+ *
+ *   void hnc3d_solute_solve (const ProblemData *PD,
+ *                            const int m, const Site solvent[m],
+ *                            const int n, const Site solute[n],
+ *                            void (*density)(int, const real[][3], real[]),
+ *                            SCM *dict, Vec g[m], const real *chi_fft_buf,
+ *                            Context **medium, Restart **restart);
+ *                                                  (hnc3d.h:13-21)
+ *
+ * PETSc Vecs, Guile SCM values and the global option fluid are replaced by
+ * plain pointers, sizes and an explicit options struct.  No torch types.
+ * All floating point is IEEE double on the boundary.
+ */
+#ifndef RISM3D_H
+#define RISM3D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Closure numbering of ClosureEnum (bgy3d.h:211-223, foreign.f90:42-57). */
+enum rism3d_closure {
+  RISM3D_CLOSURE_HNC = 0,
+  RISM3D_CLOSURE_KH = 1,
+  RISM3D_CLOSURE_PY = 2,
+  RISM3D_CLOSURE_PSE0 = 3,
+  RISM3D_CLOSURE_PSE1 = 4,
+  RISM3D_CLOSURE_PSE2 = 5,
+  RISM3D_CLOSURE_PSE3 = 6,
+  RISM3D_CLOSURE_PSE4 = 7,
+  RISM3D_CLOSURE_PSE5 = 8,
+  RISM3D_CLOSURE_PSE6 = 9,
+  RISM3D_CLOSURE_PSE7 = 10
+};
+
+/* Field-for-field the reference ProblemData (bgy3d.h:228-256).  L is the
+   full box edge and h = L/N, as bgy3d-guile.c:284-312 derives them from
+   the CLI half-width --L. */
+typedef struct rism3d_problem {
+  int N[3];          /* grid size; N[0] is the stride-1 (x) axis        */
+  double L[3];       /* box edge [A]                                    */
+  double h[3];       /* mesh width [A], L/N                             */
+  double rmax;       /* radial extent of the 1D chi table [A]           */
+  int nrad;          /* number of 1D radial points                      */
+  double beta;       /* inverse temperature [mol/kcal]                  */
+  double rho;        /* solvent site number density [A^-3]              */
+  double lambda;     /* mixing parameter of the fixed-point solvers     */
+  double damp;       /* unused on this path, kept for layout            */
+  int max_iter;      /* iteration limit                                 */
+  double norm_tol;   /* convergence threshold, max-norm of the residual */
+  int closure;       /* enum rism3d_closure                             */
+} rism3d_problem;
+
+/* Field-for-field the reference Site (bgy3d-solutes.h:9-16); the Guile
+   SCM handle becomes an opaque user pointer at the same offset. */
+typedef struct rism3d_site {
+  char name[5];
+  double x[3];       /* position [A]                                    */
+  double sigma;      /* LJ sigma [A]                                    */
+  double epsilon;    /* LJ epsilon [kcal/mol]                           */
+  double charge;     /* [e]                                             */
+  void *user;
+} rism3d_site;
+
+enum rism3d_status {
+  RISM3D_OK = 0,
+  RISM3D_NONCONVERGED = 1,  /* max_iter reached (bgy3d-snes.c:339-345)  */
+  RISM3D_NAN = 2,           /* residual not finite (diverged closure)   */
+  RISM3D_BAD_ARG = 3,
+  RISM3D_NO_DEVICE = 4,
+  RISM3D_OOM = 5,
+  RISM3D_UNSUPPORTED = 6    /* grid size or site count outside limits   */
+};
+
+enum rism3d_solver {
+  RISM3D_SOLVER_MDIIS = 0,  /* default: MDIIS / Anderson mixing         */
+  RISM3D_SOLVER_PICARD = 1  /* x += lambda F(x) (bgy3d-snes.c:348-384)  */
+};
+
+enum rism3d_precision {
+  RISM3D_FP64 = 0,          /* every kernel in FP64                     */
+  RISM3D_MIXED = 1          /* FP32 FFT/closure/contraction, FP64 state,
+                               residual, DIIS and mu; FP64 polish        */
+};
+
+/* The reference reads these from the Guile *settings* fluid
+   (hnc3d.c:1575, 1600, 1783-1784; bgy3d-snes.c:20).  Here they are
+   explicit.  Zero-initialise and call rism3d_options_default(). */
+typedef struct rism3d_options {
+  int renorm;          /* 1: tau renormalisation (default), 0: --no-renorm */
+  int solver;          /* enum rism3d_solver                               */
+  int precision;       /* enum rism3d_precision                            */
+  int polish_iters;    /* MIXED: max FP64 iterations after the FP32 phase  */
+  double mixed_tol;    /* MIXED: max-norm residual at which FP32 hands over */
+  int diis_depth;      /* MDIIS history length (1..16)                     */
+  double diis_step;    /* MDIIS step lambda applied to residuals           */
+  int verbose;         /* print per-iteration norms to stderr              */
+  int device;          /* HIP device ordinal                               */
+  double dielectric;   /* chi_k == NULL only: DRISM target epsilon of the
+                          internal 1D chi producer ("dielectric" setting,
+                          rism.f90:346); 0 selects plain RISM              */
+} rism3d_options;
+
+typedef struct rism3d_result {
+  double mu;           /* excess chemical potential, native closure [kcal/mol]
+                          ('free-energy, hnc3d.c:2232-2237)                */
+  double mu_hnc, mu_kh, mu_gf;  /* print_chempot functionals (hnc3d.c:669) */
+  double pmv_factor;   /* 1 - rho int c d3r ('partial-molar-volume-factor) */
+  double residual;     /* final max-norm of F(t)                            */
+  int iterations;      /* residual evaluations, all precisions              */
+  int iterations_fp32; /* of which in FP32                                  */
+  int status;          /* enum rism3d_status                                */
+  double *g;           /* in: NULL or caller buffer [m][N2][N1][N0]; out:
+                          g_i = h_i + 1 on the reference grid, r = i h - L/2 */
+  double *t;           /* in: NULL or caller buffer [m][N2][N1][N0]; out:
+                          converged indirect correlation t_i                */
+} rism3d_result;
+
+/* Opaque restart token: the converged t (hnc3d.c:1826-1845, 2299-2316). */
+typedef struct rism3d_restart rism3d_restart;
+
+void rism3d_options_default(rism3d_options *opt);
+
+/* Replaces hnc3d_solute_solve (hnc3d.c:1762-2350).
+   chi_k: double[m][m][nrad], solvent susceptibility chi(k) (not offset by
+   one) on k_j = (j + 1/2) pi / rmax, as produced by rism_solvent
+   (rism.f90:57-102) and checked at bgy3d-guile.c:1136-1147; or NULL to
+   compute it with the 1D DRISM producer on the "upscaled" grid
+   rmax = 2 max(L), nrad = 16 max(N) (hnc3d.c:772-783, 1471-1495).
+   restart: NULL, or in/out token (consumed on entry, replaced on exit).
+   The density callback of the reference is not supported (it aborts on
+   the HNC3D path in the reference too, bgy3d-solutes.c:964-968). */
+int rism3d_solute_solve(const rism3d_problem *pd,
+                        int m, const rism3d_site *solvent,
+                        int n, const rism3d_site *solute,
+                        const double *chi_k,
+                        const rism3d_options *opt,
+                        rism3d_result *res,
+                        rism3d_restart **restart);
+
+/* Many solutes sharing one solvent, chi and grid, solved together: every
+   kernel launch covers all unconverged solutes (one launch per pass).
+   solutes: n_solutes arrays of sites, n_sites[b] sites each.
+   res[b] receives each result. */
+int rism3d_batch_solve(const rism3d_problem *pd,
+                       int m, const rism3d_site *solvent,
+                       int n_solutes, const int *n_sites,
+                       const rism3d_site *const *solutes,
+                       const double *chi_k,
+                       const rism3d_options *opt,
+                       rism3d_result *res);
+
+/* One residual evaluation F(t) = (chi - 1) * c(t) - t, replacing
+   iterate_t1 (hnc3d.c:1184-1285) as called by the reference's SNES.
+   t, r: double[m][N2][N1][N0].  Uses the FP32 kernels when
+   opt->precision == RISM3D_MIXED (t and r stay FP64).  chi_k required. */
+int rism3d_residual(const rism3d_problem *pd,
+                    int m, const rism3d_site *solvent,
+                    int n, const rism3d_site *solute,
+                    const double *chi_k,
+                    const rism3d_options *opt,
+                    const double *t, double *r);
+
+/* Replaces bgy3d_restart_destroy (bgy3d-impure.c:593-600). */
+void rism3d_restart_destroy(rism3d_restart *restart);
+
+/* 1D DRISM solvent susceptibility, replaces rism_solvent
+   (rism.f90:57-102) with t_buf == NULL: chi_out is double[m][m][nrad].
+   dielectric == 0 selects plain XRISM (rism.f90:346-350). */
+int rism3d_solvent_chi(const rism3d_problem *pd,
+                       int m, const rism3d_site *solvent,
+                       double dielectric,
+                       double *chi_out);
+
+/* Library build/version string, for diagnostics. */
+const char *rism3d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RISM3D_H */

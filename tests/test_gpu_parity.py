@@ -1,0 +1,200 @@
+"""GPU parity: the gfx950 kernels through the C ABI against the oracle.
+
+Every case prints its name first so a failing GPU run names it.  Inputs are
+seeded; chi comes from the committed fixture (oracle 1D DRISM, README
+settings).  Tolerances: FP64 residuals 1e-9 absolute (FFT rounding on
+|t| ~ 10); converged mu 1e-8 kcal/mol and g 1e-8 max-abs at 1e-12
+residual; MIXED (FP32 bulk) mu within 0.01 kcal/mol of FP64.
+"""
+import numpy as np
+import pytest
+
+from conftest import README
+from oracle import closures as C
+from oracle import rism3d as O
+
+pytestmark = pytest.mark.gpu
+
+W = "water, PR-SPC/E"
+
+
+def _R():
+    from bgy3d_amd import rism3d as R
+    return R
+
+
+def _sites(name):
+    from bgy3d_amd.solutes import find_molecule
+    return find_molecule(name)
+
+
+def _settings(N, L=10.0, closure="HNC", **kw):
+    s = dict(README, N=N, L=L, rmax=40.0, nrad=1536, closure=closure, **kw)
+    return s
+
+
+def _oracle_problem(s):
+    N = s["N"]
+    pd = O.Problem.from_cli(s["L"], N, s["beta"], s["rho"], closure=C.NAMES[s["closure"]],
+                            rmax=s["rmax"], nrad=s["nrad"])
+    for i, k in enumerate(("nx", "ny", "nz")):
+        if k in s:
+            n = list(pd.N); n[i] = s[k]; pd.N = tuple(n)
+    for i, k in enumerate(("sx", "sy", "sz")):
+        if k in s:
+            l = list(pd.L); l[i] = s[k]; pd.L = tuple(l)
+    pd.h = tuple(l / n for l, n in zip(pd.L, pd.N))
+    return pd
+
+
+def _smooth_t(pd, m, seed):
+    rng = np.random.default_rng(seed)
+    x, y, z = O.r_axes(pd)
+    r2 = z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2
+    t = np.stack([rng.uniform(-1, 1) * np.exp(-r2 / rng.uniform(4, 16)) for _ in range(m)])
+    return t + 0.01 * rng.standard_normal(t.shape)
+
+
+RESID_CASES = [
+    # name, N (or dict), closure, renorm, solute
+    ("n32-hnc-renorm", 32, "HNC", True, W),
+    ("n48-kh-renorm-radix3", 48, "KH", True, W),
+    ("n64-pse3-norenorm", 64, "PSE3", False, W),
+    ("n32-py-renorm", 32, "PY", True, W),
+    ("n32-na-kh-charged", 32, "KH", True, "Na+, JC08, SPC/E"),
+    ("noncubic-32x48x64", {"N": 32, "ny": 48, "nz": 64, "sy": 24.0, "sz": 28.0}, "HNC", True, W),
+]
+
+
+@pytest.mark.parametrize("name,N,closure,renorm,solute", RESID_CASES, ids=[c[0] for c in RESID_CASES])
+def test_residual_fp64(name, N, closure, renorm, solute, chi_w):
+    print(f"case residual-fp64 {name}", flush=True)
+    R = _R()
+    if isinstance(N, dict):
+        s = _settings(N["N"], closure=closure, **{k: v for k, v in N.items() if k != "N"})
+    else:
+        s = _settings(N, closure=closure)
+    if not renorm:
+        s["no-renorm"] = True
+    pd = _oracle_problem(s)
+    ref = O.Solute3D(pd, _sites(W), _sites(solute), chi_w, renorm=renorm)
+    t = _smooth_t(pd, 3, seed=sum(name.encode()))
+    r_ref = ref.iterate(t)
+    r_gpu = R.residual(solute, W, s, chi_w, t)
+    err = np.max(np.abs(r_gpu - r_ref))
+    scale = np.max(np.abs(r_ref))
+    print(f"  max|dF| = {err:.3e}  max|F| = {scale:.3e}", flush=True)
+    assert err < 1e-9 * max(1.0, scale)
+
+
+def test_residual_mixed(chi_w):
+    print("case residual-mixed n64-hnc", flush=True)
+    R = _R()
+    s = _settings(64, precision="mixed")
+    pd = _oracle_problem(s)
+    ref = O.Solute3D(pd, _sites(W), _sites(W), chi_w)
+    t = _smooth_t(pd, 3, seed=7)
+    r_ref = ref.iterate(t)
+    r_gpu = R.residual(W, W, s, chi_w, t)
+    err = np.max(np.abs(r_gpu - r_ref))
+    print(f"  max|dF| = {err:.3e}", flush=True)
+    assert err < 2e-5 * max(1.0, np.max(np.abs(r_ref)))
+
+
+SOLVE_CASES = [
+    ("n32-hnc", 32, "HNC", True, W),
+    ("n48-kh", 48, "KH", True, W),
+    ("n32-pse3", 32, "PSE3", True, W),
+    ("n32-hnc-norenorm", 32, "HNC", False, W),
+    ("n32-na-kh", 32, "KH", True, "Na+, JC08, SPC/E"),
+    ("n32-cl-pse2", 32, "PSE2", True, "Cl-, JC08, SPC/E"),
+]
+
+
+@pytest.mark.parametrize("name,N,closure,renorm,solute", SOLVE_CASES, ids=[c[0] for c in SOLVE_CASES])
+def test_solve_fp64(name, N, closure, renorm, solute, chi_w):
+    print(f"case solve-fp64 {name}", flush=True)
+    R = _R()
+    s = _settings(N, closure=closure, **{"norm-tol": 1e-12, "max-iter": 2000})
+    if not renorm:
+        s["no-renorm"] = True
+    pd = _oracle_problem(s)
+    ref = O.solute_solve(pd, _sites(W), _sites(solute), chi_w, renorm=renorm, atol=1e-12)
+    out = R.hnc3d_run_solute(solute, W, s, chi=chi_w)
+    dmu = abs(out["free-energy"] - ref["mu"])
+    dg = np.max(np.abs(out["GUV"] - ref["g"]))
+    print(f"  mu gpu {out['free-energy']:.10f} oracle {ref['mu']:.10f} |dmu| {dmu:.2e} "
+          f"max|dg| {dg:.2e} iters {out['iterations']}", flush=True)
+    assert out["status"] == "OK"
+    assert dmu < 1e-8
+    assert dg < 1e-8
+    for key, ok in (("free-energy-hnc", "mu_hnc"), ("free-energy-kh", "mu_kh"),
+                    ("free-energy-gf", "mu_gf")):
+        assert abs(out[key] - ref[ok]) < 1e-8
+    assert abs(out["partial-molar-volume-factor"] - ref["pmv_factor"]) < 1e-8
+
+
+def test_readme_anchor_96(chi_w):
+    """README:20-31: mu = -6.349052 (renorm), -6.297170 (--no-renorm);
+    oracle values -6.3490515257 / -6.2971695759 on the same chi."""
+    print("case readme-96", flush=True)
+    R = _R()
+    for renorm, readme, oracle_mu in ((True, -6.349052, -6.3490515257139934),
+                                      (False, -6.297170, -6.297169575901992)):
+        s = _settings(96, **{"norm-tol": 1e-12, "max-iter": 2000})
+        if not renorm:
+            s["no-renorm"] = True
+        out = R.hnc3d_run_solute(W, W, s, chi=chi_w)
+        mu = out["free-energy"]
+        print(f"  renorm={renorm} mu {mu:.10f} README {readme} iters {out['iterations']}", flush=True)
+        assert abs(mu - readme) < 1e-6
+        assert abs(mu - oracle_mu) < 1e-8
+
+
+def test_mixed_vs_fp64(chi_w):
+    print("case mixed-vs-fp64 n64", flush=True)
+    R = _R()
+    s = _settings(64, **{"norm-tol": 1e-10, "max-iter": 2000})
+    f64 = R.hnc3d_run_solute(W, W, s, chi=chi_w)
+    for polish in (0, 3):
+        m = R.hnc3d_run_solute(W, W, dict(s, precision="mixed", **{"polish-iters": polish}), chi=chi_w)
+        dmu = abs(m["free-energy"] - f64["free-energy"])
+        dg = np.max(np.abs(m["GUV"] - f64["GUV"]))
+        print(f"  polish {polish}: |dmu| {dmu:.2e} max|dg| {dg:.2e} iters {m['iterations']} "
+              f"fp32 {m['iterations-fp32']}", flush=True)
+        assert m["status"] == "OK"
+        assert dmu < 0.01
+
+
+def test_batch_equals_single(chi_w):
+    print("case batch-vs-single n32", flush=True)
+    R = _R()
+    s = _settings(32, **{"norm-tol": 1e-11, "max-iter": 2000})
+    solutes = [W, "Na+, JC08, SPC/E", "Cl-, JC08, SPC/E"]
+    batch = R.batch_solve(solutes, W, s, chi_w)
+    for name, b in zip(solutes, batch):
+        one = R.hnc3d_run_solute(name, W, s, chi=chi_w)
+        print(f"  {name}: batch {b['free-energy']:.10f} single {one['free-energy']:.10f}", flush=True)
+        assert b["status"] == "OK"
+        assert abs(b["free-energy"] - one["free-energy"]) < 1e-9
+
+
+def test_restart_token(chi_w):
+    print("case restart n32", flush=True)
+    R = _R()
+    s = _settings(32, **{"norm-tol": 1e-11, "max-iter": 2000})
+    a = R.hnc3d_run_solute(W, W, s, chi=chi_w, restart=0)
+    tok = a["RESTART"]
+    assert tok
+    b = R.hnc3d_run_solute(W, W, s, chi=chi_w, restart=tok)
+    print(f"  cold iters {a['iterations']} warm iters {b['iterations']}", flush=True)
+    assert b["iterations"] < a["iterations"]
+    assert abs(a["free-energy"] - b["free-energy"]) < 1e-9
+    R.restart_destroy(b["RESTART"])
+
+
+def test_nonconverged_status(chi_w):
+    print("case max-iter-1", flush=True)
+    R = _R()
+    out = R.hnc3d_run_solute(W, W, _settings(32, **{"max-iter": 1}), chi=chi_w)
+    assert out["status"] == "NONCONVERGED"
