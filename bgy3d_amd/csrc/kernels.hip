@@ -134,6 +134,18 @@ fft_y(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   }
 }
 
+// |k|^2 = (kx^2 + ky^2) + kz^2, rounded exactly as the reference's
+// SQR(k[0]) + SQR(k[1]) + SQR(k[2]) (bgy3d-vec.h:664-667): no FMA
+// contraction.  interp() below is discontinuous at table midpoints and grid
+// k often lands exactly on one, so the last bit selects the neighbour.
+__device__ __forceinline__ double kmag2(double kx, double ky, double kz) {
+#pragma clang fp contract(off)
+  const double a = kx * kx;
+  const double b = ky * ky;
+  const double c = kz * kz;
+  return (a + b) + c;
+}
+
 // 1D table lookup (interp, bgy3d-vec.c:23-46) at |k|; the index set is
 // shared by every table at one k-point.
 struct Interp {
@@ -193,8 +205,8 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     if (kx0 + l >= g.NH) continue;
     const int iz = (z <= N2 / 2) ? z : z - N2;
     const double kx = (kx0 + l) * g.dkx, kz = iz * g.dkz;
-    const double k = sqrt(kx * kx + ky * ky + kz * kz);
-    const Interp ip = interp_at(k, g.dk_tab, g.nrad);
+    const double k2 = kmag2(kx, ky, kz);
+    const Interp ip = interp_at(sqrt(k2), g.dk_tab, g.nrad);
     const cpx<T> fs = Fs[z * zs + l];
     cpx<T> c[MAX_SITES];
     for (int s = 0; s < m; ++s) c[s] = lds[(s * LINES + l) * LS + z];
@@ -202,7 +214,6 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     if (!g.renorm) {
       // u_c(k) = 4 pi eps0^-1 exp(-k^2/4G^2)/k^2 F(k) s(k), 0 at k = 0
       // (bgy3d-force.h:140-147, bgy3d-solutes.c:767-792).
-      const double k2 = kx * kx + ky * ky + kz * kz;
       const double shape = (k2 == 0.0) ? 0.0
           : (4 * M_PI * EPSILON0INV) * exp(-k2 / (4 * G_COULOMB * G_COULOMB)) / k2;
       uc = cscale(fs, T(shape));

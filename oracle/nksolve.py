@@ -19,6 +19,43 @@ class NotConverged(RuntimeError):
     pass
 
 
+def mdiis(F, x0, atol, depth=10, lam=0.5, max_iter=3000):
+    """Pulay/MDIIS fixed-point iteration x <- sum c_j (x_j + lam F(x_j)),
+    used to bring Newton into its basin on coarse grids.  Returns (x, it)."""
+    xs, rs = [], []
+    x = np.array(x0, dtype=np.float64, copy=True)
+    best = np.inf
+    for it in range(max_iter):
+        r = F(x)
+        rn = np.max(np.abs(r))
+        if not np.isfinite(rn):
+            raise NotConverged("MDIIS residual is not finite")
+        if rn <= atol:
+            return x, it
+        r2 = float(np.vdot(r, r))
+        if r2 > 100 * best:
+            xs, rs = [], []
+        best = min(best, r2)
+        xs.append(x)
+        rs.append(r)
+        if len(xs) > depth:
+            xs.pop(0)
+            rs.pop(0)
+        k = len(rs)
+        A = np.zeros((k + 1, k + 1))
+        A[:k, :k] = [[float(np.vdot(a, b)) for b in rs] for a in rs]
+        A[:k, :k] /= np.max(np.abs(np.diag(A[:k, :k])))
+        A[k, :k] = A[:k, k] = -1.0
+        rhs = np.zeros(k + 1)
+        rhs[k] = -1.0
+        try:
+            c = np.linalg.solve(A, rhs)[:k]
+        except np.linalg.LinAlgError:
+            xs, rs, c = [xs[-1]], [rs[-1]], np.array([1.0])
+        x = sum(ci * (xi + lam * ri) for ci, xi, ri in zip(c, xs, rs))
+    raise NotConverged(f"MDIIS: no convergence after {max_iter} iterations")
+
+
 def newton_krylov(F, x0, J=None, atol=1e-12, max_iter=100, ksp_rtol=1e-4,
                   restart=30, ksp_maxiter=300, verbose=False):
     """Solve F(x) = 0 from x0.  Stops when max|F| <= atol.

@@ -25,9 +25,12 @@ irfftn(norm="forward"), which shares FFTW's treatment of the kx = 0 and
 kx = N0/2 planes (complex inverse over z, y, then real part at DC/Nyquist).
 
 Pinning: README:20-31 (-6.349052 renorm, -6.297170 --no-renorm for
-PR-SPC/E water at N=96, L=10, chi from 1D DRISM at rmax 10, nrad 96),
-test/out/run-ions-out:56-110 (charged single sites, KH/PSE-n/HNC), plus
-grid invariants (translation by whole cells), in tests/test_oracle_*.py.
+PR-SPC/E water at --L 10 --N 96) reproduced to 1e-7 with chi from 1D DRISM
+on the grid hnc3d_solute_solve builds itself when chi_fft_buf is NULL
+(upscale(): rmax = 40, nrad = 1536, hnc3d.c:772-783);
+test/out/run-ions-out:56-110 (charged single sites, KH/PSE-n/HNC) to the
+reference's own reproducibility floor; grid invariants (translation by whole
+cells); in tests/test_oracle_*.py.
 """
 import math
 
@@ -36,7 +39,7 @@ from scipy.special import erf, erfc
 
 from . import closures as C
 from . import rism1d
-from .nksolve import newton_krylov
+from .nksolve import NotConverged, mdiis, newton_krylov
 
 EPSILON0INV = 331.84164
 G = 1.2
@@ -90,8 +93,15 @@ def k_axes(pd):
 
 
 def kmag(pd):
+    """|k| = sqrt((kx^2 + ky^2) + kz^2) in the reference's order
+    (SQR(k[0]) + SQR(k[1]) + SQR(k[2]), bgy3d-vec.h:664-667).  The order
+    matters: interp (bgy3d-vec.c:23-46) is discontinuous at table midpoints
+    and grid k often lands exactly on one (|k|/dk_tab integer), so the
+    last bit of |k|^2 picks the neighbour."""
     kx, ky, kz = k_axes(pd)
-    return np.sqrt(kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    k2 = (kx[None, None, :] * kx[None, None, :] + ky[None, :, None] * ky[None, :, None]) \
+        + kz[:, None, None] * kz[:, None, None]
+    return np.sqrt(k2)
 
 
 def phase_sign(pd):
@@ -289,10 +299,18 @@ class Solute3D:
         return out if flag else out - dt
 
     def solve(self, t0=None, atol=1e-11, verbose=False):
+        """MDIIS to atol; if it stalls, Newton-GMRES from its best iterate.
+        Only the root matters for parity (see nksolve)."""
         t0 = np.zeros_like(self.v) if t0 is None else t0
+        try:
+            t, n_pre = mdiis(self.iterate, t0, atol=atol, max_iter=1500)
+            return t, {"n_pre": n_pre, "iters": 0, "n_f": n_pre, "n_j": 0}
+        except NotConverged:
+            t0, n_pre = mdiis(self.iterate, t0, atol=1e-4)
         jac = self.jacobian if self.pd.closure != C.PY else None
         t, stats = newton_krylov(self.iterate, t0, J=jac, atol=atol,
                                  verbose=verbose, ksp_rtol=1e-3)
+        stats["n_pre"] = n_pre
         return t, stats
 
     def observables(self, t):

@@ -1,0 +1,116 @@
+"""The C ABI without a GPU: the library loads, exports every entry point
+include/rism3d.h declares, its structs have the layout the header defines
+(checked against gcc on the header itself), argument validation runs before
+any device call, and compute calls fail loudly (NO_DEVICE) without a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "rism3d.h")
+
+
+def _R():
+    from bgy3d_amd import rism3d as R
+    return R
+
+
+def test_exports_every_declared_symbol():
+    R = _R()
+    lib = R.load()
+    text = open(HEADER).read()
+    declared = sorted(set(re.findall(r"\b(rism3d_[a-z_0-9]+)\s*\(", text)))
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), f"{name} declared in rism3d.h but not exported"
+    assert set(R.EXPORTS) <= set(declared)
+
+
+def test_struct_layout_matches_header(tmp_path):
+    R = _R()
+    prog = tmp_path / "layout.c"
+    fields = {
+        "rism3d_problem": [f for f, _ in R.Problem._fields_],
+        "rism3d_site": [f for f, _ in R.Site._fields_],
+        "rism3d_options": [f for f, _ in R.Options._fields_],
+        "rism3d_result": [f for f, _ in R.Result._fields_],
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    for st, fs in fields.items():
+        lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for f in fs:
+            cf = "lambda" if f == "lambda_" else f
+            lines.append(f'printf("{st}.{f} %zu\\n", offsetof({st}, {cf}));')
+    lines.append("return 0;}")
+    prog.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", str(prog), "-o", str(exe)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                        text=True).stdout.split("\n") if l)
+    py = {"rism3d_problem": R.Problem, "rism3d_site": R.Site, "rism3d_options": R.Options,
+          "rism3d_result": R.Result}
+    for st, cls in py.items():
+        assert int(out[st]) == ctypes.sizeof(cls), st
+        for f, _ in cls._fields_:
+            assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)
+
+
+def test_problem_layout_is_reference_problem_data():
+    """ProblemData (bgy3d.h:228-256): int N[3]; double L[3], h[3], rmax;
+    int nrad; double beta, rho, lambda, damp; int max_iter; double norm_tol;
+    ClosureEnum closure -- same order and types."""
+    R = _R()
+    names = [f for f, _ in R.Problem._fields_]
+    assert names == ["N", "L", "h", "rmax", "nrad", "beta", "rho", "lambda_", "damp",
+                     "max_iter", "norm_tol", "closure"]
+
+
+def test_problem_data_semantics():
+    R = _R()
+    pd = R.problem_data({"L": 10, "N": 96})
+    assert list(pd.N) == [96] * 3 and list(pd.L) == [20.0] * 3      # --L is the half edge
+    assert abs(pd.h[0] - 20.0 / 96) < 1e-15
+    assert pd.rmax == 10.0 and pd.nrad == 96                         # bgy3d-guile.c:328-331
+    assert pd.closure == 0 and pd.max_iter == 1500
+    pd = R.problem_data({"L": 10, "N": 64, "closure": "PSE3", "nrad": 4096, "rmax": 160.0})
+    assert pd.closure == 6 and pd.nrad == 4096 and pd.rmax == 160.0
+
+
+def _chi(m=3, nrad=1536):
+    return np.tile(np.eye(m)[:, :, None], (1, 1, nrad))
+
+
+def _gpu_present():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+def test_validation_before_device():
+    R = _R()
+    W = "water, PR-SPC/E"
+    base = {"L": 10, "N": 32, "rmax": 40.0, "nrad": 1536}
+    with pytest.raises(R.RismError) as e:
+        R.hnc3d_run_solute(W, W, dict(base, N=100), chi=_chi())
+    assert e.value.status == R.UNSUPPORTED
+    with pytest.raises(R.RismError) as e:  # k table too short (interp asserts)
+        R.hnc3d_run_solute(W, W, dict(base, nrad=16), chi=_chi(nrad=16))
+    assert e.value.status == R.BAD_ARG
+    with pytest.raises(ValueError):
+        R.hnc3d_run_solute(W, W, base, chi=_chi(nrad=100))
+
+
+@pytest.mark.skipif(_gpu_present(), reason="checks the no-GPU error path")
+def test_no_device_fails_loudly():
+    R = _R()
+    W = "water, PR-SPC/E"
+    with pytest.raises(R.RismError) as e:
+        R.hnc3d_run_solute(W, W, {"L": 10, "N": 32, "rmax": 40.0, "nrad": 1536}, chi=_chi())
+    assert e.value.status == R.NO_DEVICE
