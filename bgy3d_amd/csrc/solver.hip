@@ -252,6 +252,19 @@ struct Problem {
   std::vector<SoluteState> st;
 };
 
+// Per-kernel HIP-event timing of the residual chain (rism3d_profile_*).
+// Kernel ids: 0 closure_r2c_x, 1 fft_y (forward), 2 zstar, 3 fft_y
+// (inverse), 4 c2r_x_resid.  Bytes are the algorithmic (compulsory) HBM
+// traffic of each launch: every array the kernel must read or write, once.
+struct Profile {
+  bool on = false;
+  hipEvent_t ev[5][2] = {};
+  double ms[5] = {0, 0, 0, 0, 0};
+  double bytes[5] = {0, 0, 0, 0, 0};
+  long launches[5] = {0, 0, 0, 0, 0};
+};
+Profile g_prof;
+
 int resid_grid_x(const GridArgs& g) {
   const int rows = rows_for(g.N0 / 2);
   return (g.N1 * g.N2 + rows - 1) / rows;
@@ -261,26 +274,50 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
   const GridArgs& g = pb.g;
   HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), sd.size() * sizeof(SoluteDesc), hipMemcpyHostToDevice, pb.s));
   const dim3 gx(resid_grid_x(g), g.m, pb.B);
+  Profile& pf = g_prof;
+  auto mark = [&](int k, int which) {
+    if (pf.on) HIPCHK(hipEventRecord(pf.ev[k][which], pb.s));
+  };
   if (fp32) {
     using T = float;
     constexpr int LN = y_lines<T>();
     const dim3 gy((g.NH + LN - 1) / LN, g.m * g.N2, pb.B);
-    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p);
-    launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p);
-    launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi_tab.p, pb.tau_tab.p);
-    launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p);
-    launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p);
+    mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p); mark(0, 1);
+    mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(1, 1);
+    mark(2, 0); launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi_tab.p, pb.tau_tab.p); mark(2, 1);
+    mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(3, 1);
+    mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p); mark(4, 1);
   } else {
     using T = double;
     constexpr int LN = y_lines<T>();
     const dim3 gy((g.NH + LN - 1) / LN, g.m * g.N2, pb.B);
-    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
-    launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
-    launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi_tab.p, pb.tau_tab.p);
-    launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
-    launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
+    mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p); mark(0, 1);
+    mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(1, 1);
+    mark(2, 0); launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi_tab.p, pb.tau_tab.p); mark(2, 1);
+    mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(3, 1);
+    mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p); mark(4, 1);
   }
   hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, pb.d_sd.p, pb.nparts, pb.d_out.p);
+  if (pf.on) {
+    int na = 0;
+    for (auto& d : sd) na += d.active != 0;
+    const double sc = fp32 ? 4.0 : 8.0;  // working-type scalar
+    const double mP = (double)g.m * g.P, mK = (double)g.m * g.K;
+    const double per[5] = {
+        mP * 8 + mP * sc + mK * 2 * sc,       // t, beta v in; C out
+        2 * mK * 2 * sc,                      // W in/out
+        2 * mK * 2 * sc + (double)g.K * 2 * sc,  // W in/out, F s in
+        2 * mK * 2 * sc,
+        mK * 2 * sc + mP * 8 + mP * 8};       // W, t in; r out
+    HIPCHK(hipEventSynchronize(pf.ev[4][1]));
+    for (int k = 0; k < 5; ++k) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, pf.ev[k][0], pf.ev[k][1]));
+      pf.ms[k] += ms;
+      pf.bytes[k] += per[k] * na;
+      pf.launches[k] += 1;
+    }
+  }
 }
 
 struct Config {
@@ -893,5 +930,30 @@ int rism3d_solvent_chi(const rism3d_problem* pd, int m, const rism3d_site* solve
 }
 
 const char* rism3d_version(void) { return "rism3d-mi355x 0.1 (gfx950)"; }
+
+int rism3d_profile_start(int device) {
+  if (hipSetDevice(device) != hipSuccess) return RISM3D_NO_DEVICE;
+  auto& pf = rism::g_prof;
+  for (int k = 0; k < 5; ++k) {
+    for (int w = 0; w < 2; ++w)
+      if (!pf.ev[k][w] && hipEventCreate(&pf.ev[k][w]) != hipSuccess) return RISM3D_NO_DEVICE;
+    pf.ms[k] = 0;
+    pf.bytes[k] = 0;
+    pf.launches[k] = 0;
+  }
+  pf.on = true;
+  return RISM3D_OK;
+}
+
+int rism3d_profile_read(double* ms, double* bytes, long* launches) {
+  auto& pf = rism::g_prof;
+  for (int k = 0; k < 5; ++k) {
+    if (ms) ms[k] = pf.ms[k];
+    if (bytes) bytes[k] = pf.bytes[k];
+    if (launches) launches[k] = pf.launches[k];
+  }
+  pf.on = false;
+  return RISM3D_OK;
+}
 
 }  // extern "C"

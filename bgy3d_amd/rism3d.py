@@ -78,7 +78,9 @@ class Result(ctypes.Structure):
 
 EXPORTS = ("rism3d_options_default", "rism3d_solute_solve", "rism3d_batch_solve",
            "rism3d_residual", "rism3d_restart_destroy", "rism3d_solvent_chi",
-           "rism3d_version")
+           "rism3d_version", "rism3d_profile_start", "rism3d_profile_read")
+
+KERNELS = ("closure_r2c_x", "fft_y_fwd", "zstar", "fft_y_inv", "c2r_x_resid")
 
 _lib = None
 
@@ -111,6 +113,10 @@ def load():
         lib.rism3d_solvent_chi.restype = ctypes.c_int
         lib.rism3d_version.argtypes = []
         lib.rism3d_version.restype = ctypes.c_char_p
+        lib.rism3d_profile_start.argtypes = [ctypes.c_int]
+        lib.rism3d_profile_start.restype = ctypes.c_int
+        lib.rism3d_profile_read.argtypes = [dp, dp, P(ctypes.c_long)]
+        lib.rism3d_profile_read.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -271,8 +277,8 @@ def batch_solve(solutes, solvent, settings, chi, want_g=False):
     chi = np.ascontiguousarray(chi, dtype=np.float64)
     st = lib.rism3d_batch_solve(ctypes.byref(pd), m, sv, B, ns, ptrs, _dptr(chi),
                                 ctypes.byref(opt), res)
-    if st not in (OK, NONCONVERGED):
-        raise RismError(st, "rism3d_batch_solve")
+    if st not in (OK, NONCONVERGED, NAN):
+        raise RismError(st, "rism3d_batch_solve")  # whole-call failures only
     out = []
     for b in range(B):
         d = {"free-energy": res[b].mu, "partial-molar-volume-factor": res[b].pmv_factor,
@@ -313,3 +319,18 @@ def rism_solvent(solvent, settings):
     if st != OK:
         raise RismError(st, "rism3d_solvent_chi")
     return {"susceptibility": chi}
+
+
+def profile_start(device=0):
+    st = load().rism3d_profile_start(device)
+    if st != OK:
+        raise RismError(st, "rism3d_profile_start")
+
+
+def profile_read():
+    """{kernel: (ms, algorithmic bytes, launches)} since profile_start."""
+    ms = np.zeros(5)
+    by = np.zeros(5)
+    n = (ctypes.c_long * 5)()
+    load().rism3d_profile_read(_dptr(ms), _dptr(by), n)
+    return {k: (float(ms[i]), float(by[i]), int(n[i])) for i, k in enumerate(KERNELS)}

@@ -178,6 +178,17 @@ int rism3d_solvent_chi(const rism3d_problem *pd,
                        double dielectric,
                        double *chi_out);
 
+/* Kernel timing (no counterpart in the reference, which has no in-code
+   profiling; for benchmarks).  After rism3d_profile_start, every residual
+   evaluation brackets its five kernels with HIP events on the solver's
+   stream:  0 closure_r2c_x, 1 fft_y forward, 2 zstar (z FFTs + chi
+   contraction), 3 fft_y inverse, 4 c2r_x_resid.  rism3d_profile_read
+   returns, per kernel, the summed event time [ms], the summed algorithmic
+   HBM bytes (each array the kernel must read or write, counted once) and
+   the launch count, then stops profiling.  Arrays of 5. */
+int rism3d_profile_start(int device);
+int rism3d_profile_read(double *ms, double *bytes, long *launches);
+
 /* Library build/version string, for diagnostics. */
 const char *rism3d_version(void);
 

@@ -27,6 +27,11 @@ FIXTURES = {
     "chi_prspce_hnc_eps78.4_rmax40_nrad1536.npy": dict(
         solvent="water, PR-SPC/E", closure=0, rmax=40.0, nrad=1536,
         beta=1.6889, rho=0.0333295, eps=78.4),
+    # The solvent of the benchmark workload: PR-SPC/E water, HNC, 1D grid
+    # rmax 160 A, nrad 4096 (the 1D run of README:9-16).
+    "chi_prspce_hnc_eps78.4_rmax160_nrad4096.npy": dict(
+        solvent="water, PR-SPC/E", closure=0, rmax=160.0, nrad=4096,
+        beta=1.6889, rho=0.0333295, eps=78.4),
     # test/run-ions.scm:16-30 settings, KH closure, cSPC/E water.
     "chi_cspce_kh_eps78.4_rmax160_nrad4096.npy": dict(
         solvent="water, cSPC/E", closure=1, rmax=160.0, nrad=4096,
