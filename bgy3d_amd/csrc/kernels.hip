@@ -72,6 +72,8 @@ __host__ __device__ constexpr int rows_for(int M) {
 }
 
 // ------------------------------------------- 1. closure fused with x r2c
+// t = sum_j coef_j u_j (the MDIIS step; written back as the iterate) or the
+// given t, then c = closure(beta v, t), then the x-axis r2c of c.
 // Grid: x = ceil(N1 N2 / ROWS), y = site, z = solute.
 template <typename T, int N0>
 __global__ void __launch_bounds__(NT)
@@ -80,18 +82,31 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
   constexpr int M = N0 / 2;
   constexpr int ROWS = rows_for(M);
   __shared__ cpx<T> lds[ROWS * M];
-  const SoluteDesc d = sd[blockIdx.z];
+  const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
   const int site = blockIdx.y;
   const int nrows = g.N1 * g.N2;
   const int row0 = blockIdx.x * ROWS;
-  const double* __restrict__ t = d.t + site * g.P + (int64_t)row0 * N0;
-  const T* __restrict__ bv = static_cast<const T*>(d.bv) + site * g.P + (int64_t)row0 * N0;
+  const int64_t off = site * g.P + (int64_t)row0 * N0;
+  T* __restrict__ t = static_cast<T*>(d.t) + off;
+  const T* __restrict__ bv = static_cast<const T*>(d.bv) + off;
+  const int nc = d.ncomb;
   T* ldsr = reinterpret_cast<T*>(lds);
   const int nvalid = min(ROWS, nrows - row0) * N0;
   for (int e = threadIdx.x; e < ROWS * N0; e += NT) {
     T c = T(0);
-    if (e < nvalid) c = closure_c<T>(g.closure, g.pse_order, bv[e], T(t[e]));
+    if (e < nvalid) {
+      T tv;
+      if (nc > 0) {
+        double s = 0.0;
+        for (int j = 0; j < nc; ++j) s += d.coef[j] * (double)static_cast<const T*>(d.u[j])[off + e];
+        tv = T(s);
+        t[e] = tv;
+      } else {
+        tv = t[e];
+      }
+      c = closure_c<T>(g.closure, g.pse_order, bv[e], tv);
+    }
     ldsr[e] = c;  // z_n = c_{2n} + i c_{2n+1}, row stride M complex
   }
   __syncthreads();
@@ -116,7 +131,7 @@ __global__ void __launch_bounds__(NT)
 fft_y(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw) {
   constexpr int LS = N1 + 1;
   __shared__ cpx<T> lds[LINES * LS];
-  const SoluteDesc d = sd[blockIdx.z];
+  const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
   const int site = blockIdx.y / g.N2;
   const int z = blockIdx.y - site * g.N2;
@@ -164,28 +179,62 @@ __device__ __forceinline__ Interp interp_at(double k, double dk, int n) {
   return r;
 }
 __device__ __forceinline__ double interp_eval(const Interp& a, const double* __restrict__ tab) {
-  const double fi = (tab[a.ip] - tab[a.im]) / (double)(a.ip - a.im);
+  const double d = tab[a.ip] - tab[a.im];
+  const double fi = (a.ip - a.im == 2) ? d * 0.5 : d;  // == d / (ip - im), exactly
   return tab[a.i] + fi * a.off;
 }
 
+// chi_ij(|k|) - delta_ij and tau_i(|k|) on the 3D k grid, once per batch
+// (solute independent; solvent_kernel_rism hnc3d.c:1443-1566 tabulates
+// them the same way with vec_ktab).  Grid: x = chunks of K.  Writes FP64
+// and FP32 copies, [npairs][K] and [m][K].
+__global__ void __launch_bounds__(NT)
+ktab3(GridArgs g, const double* __restrict__ chi_tab, const double* __restrict__ tau_tab,
+      double* __restrict__ chi3, float* __restrict__ chi3f, double* __restrict__ tau3,
+      float* __restrict__ tau3f) {
+  const int np = g.m * (g.m + 1) / 2;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.K; e += stride) {
+    const int x = (int)(e % g.NH);
+    const int y = (int)((e / g.NH) % g.N1);
+    const int z = (int)(e / ((int64_t)g.NH * g.N1));
+    const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
+    const int iz = (z <= g.N2 / 2) ? z : z - g.N2;
+    const double k2 = kmag2(x * g.dkx, iy * g.dky, iz * g.dkz);
+    const Interp ip = interp_at(sqrt(k2), g.dk_tab, g.nrad);
+    for (int p = 0; p < np; ++p) {
+      const double v = interp_eval(ip, chi_tab + (int64_t)p * g.nrad);
+      chi3[p * g.K + e] = v;
+      chi3f[p * g.K + e] = (float)v;
+    }
+    for (int i = 0; i < g.m; ++i) {
+      const double v = interp_eval(ip, tau_tab + (int64_t)i * g.nrad);
+      tau3[i * g.K + e] = v;
+      tau3f[i * g.K + e] = (float)v;
+    }
+  }
+}
+
 // --------------------------- 3. z c2c + chi contraction + tau + z c2c^-1
-// Grid: x = ceil(NH / LINES), y = ky row, z = solute.  Dynamic LDS:
-// m * LINES * (N2 + 1) complex.
-// chi_tab: [m(m+1)/2][nrad] chi_ij - delta_ij; tau_tab: [m][nrad] s_i(k).
+// Grid: x = solute (fastest: concurrent workgroups share one chi3/tau3
+// tile), y = ceil(NH / LINES), z = ky row.  Dynamic LDS:
+// m * LINES * (N2 + 1) complex.  chi3: [m(m+1)/2][K] chi_ij - delta_ij;
+// tau3: [m][K] s_i(|k|), both on the k grid, in the working type.
 template <typename T, int N2, int LINES>
 __global__ void __launch_bounds__(NT)
 zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw,
-      const double* __restrict__ chi_tab, const double* __restrict__ tau_tab) {
+      const T* __restrict__ chi3, const T* __restrict__ tau3) {
   constexpr int LS = N2 + 1;
   extern __shared__ unsigned char smem[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-  const SoluteDesc d = sd[blockIdx.z];
+  const SoluteDesc& d = sd[blockIdx.x];
   if (!d.active) return;
   const int m = g.m;
-  const int y = blockIdx.y;
-  const int kx0 = blockIdx.x * LINES;
+  const int y = blockIdx.z;
+  const int kx0 = blockIdx.y * LINES;
   const int64_t zs = (int64_t)g.N1 * g.NH;  // z stride
-  cpx<T>* W = static_cast<cpx<T>*>(d.W) + (int64_t)y * g.NH + kx0;
+  const int64_t base = (int64_t)y * g.NH + kx0;
+  cpx<T>* W = static_cast<cpx<T>*>(d.W) + base;
   for (int s = 0; s < m; ++s) {
     const cpx<T>* Ws = W + s * g.K;
     cpx<T>* L = lds + s * LINES * LS;
@@ -197,16 +246,13 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   __syncthreads();
   for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, false>(lds + s * LINES * LS, LS, 1, tw);
 
-  const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + (int64_t)y * g.NH + kx0;
+  const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + base;
   const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
   const double ky = iy * g.dky;
   for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
     const int z = e / LINES, l = e - z * LINES;
     if (kx0 + l >= g.NH) continue;
-    const int iz = (z <= N2 / 2) ? z : z - N2;
-    const double kx = (kx0 + l) * g.dkx, kz = iz * g.dkz;
-    const double k2 = kmag2(kx, ky, kz);
-    const Interp ip = interp_at(sqrt(k2), g.dk_tab, g.nrad);
+    const int64_t kk = base + z * zs + l;   // k-grid index of this point
     const cpx<T> fs = Fs[z * zs + l];
     cpx<T> c[MAX_SITES];
     for (int s = 0; s < m; ++s) c[s] = lds[(s * LINES + l) * LS + z];
@@ -214,6 +260,8 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     if (!g.renorm) {
       // u_c(k) = 4 pi eps0^-1 exp(-k^2/4G^2)/k^2 F(k) s(k), 0 at k = 0
       // (bgy3d-force.h:140-147, bgy3d-solutes.c:767-792).
+      const int iz = (z <= N2 / 2) ? z : z - N2;
+      const double k2 = kmag2((kx0 + l) * g.dkx, ky, iz * g.dkz);
       const double shape = (k2 == 0.0) ? 0.0
           : (4 * M_PI * EPSILON0INV) * exp(-k2 / (4 * G_COULOMB * G_COULOMB)) / k2;
       uc = cscale(fs, T(shape));
@@ -226,14 +274,14 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
       cpx<T> acc = {T(0), T(0)};
       for (int j = 0; j < m; ++j) {
         const int pi_ = (i >= j) ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
-        const T chi = T(interp_eval(ip, chi_tab + (int64_t)pi_ * g.nrad));
+        const T chi = chi3[pi_ * g.K + kk];
         acc.re += chi * c[j].re;
         acc.im += chi * c[j].im;
       }
       cpx<T> out;
       if (g.renorm) {
         // (h^3/L^3) sum_j chi_ij C_j - (beta/eps0/L^3) tau_i F s  (hnc3d.c:1245-1268)
-        const T tau = T(interp_eval(ip, tau_tab + (int64_t)i * g.nrad) * g.tau_scale);
+        const T tau = tau3[i * g.K + kk] * T(g.tau_scale);
         out = csub(cscale(acc, T(g.a_scale)), cscale(fs, tau));
       } else {
         out = cscale(csub(acc, cscale(uc, T(g.beta * g.q[i]))), T(g.inv_L3));
@@ -264,8 +312,9 @@ __device__ __forceinline__ double warp_max(double v) {
 }
 
 // -------------------------------------- 5. x c2r fused with the residual
-// Grid as closure_r2c_x.  Writes r = t' - t and (max|r|, sum r^2) per
-// workgroup to part[(site * gridDim.x + blockIdx.x) * 2 + {0,1}].
+// t' = c2r(...), r = t' - t, u = t + lam r, and per workgroup the partial
+// max|r|, sum r^2 and dots <r, r_j> with the history (MDIIS Gram row), to
+// part[(site * gridDim.x + blockIdx.x) * NPART + q].  Grid as closure_r2c_x.
 template <typename T, int N0>
 __global__ void __launch_bounds__(NT)
 c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
@@ -274,8 +323,8 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   constexpr int ROWS = rows_for(M);
   __shared__ cpx<T> xb[ROWS * (M + 1)];
   __shared__ cpx<T> zb[ROWS * M];
-  __shared__ double red[2][NT / 64];
-  const SoluteDesc d = sd[blockIdx.z];
+  __shared__ double red[NPART][NT / 64];
+  const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
   const int site = blockIdx.y;
   const int nrows = g.N1 * g.N2;
@@ -302,116 +351,97 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   __syncthreads();
   fft_lines<T, M, ROWS, NT, true>(zb, M, 1, tw_half);
   const T* zr = reinterpret_cast<const T*>(zb);
-  const double* __restrict__ t = d.t + site * g.P + (int64_t)row0 * N0;
-  double* __restrict__ r = d.r + site * g.P + (int64_t)row0 * N0;
+  const int64_t off = site * g.P + (int64_t)row0 * N0;
+  const T* __restrict__ t = static_cast<const T*>(d.t) + off;
+  T* __restrict__ r = static_cast<T*>(d.r) + off;
+  T* __restrict__ un = static_cast<T*>(d.unew) + off;
+  const int nd = d.ndot;
+  const T lam = T(d.lam);
   double mx = 0.0, ss = 0.0;
-  for (int e = threadIdx.x; e < vrows * N0; e += NT) {
-    const double rv = (double)zr[e] - t[e];
-    r[e] = rv;
-    mx = fmax(mx, fabs(rv));
-    ss += rv * rv;
-  }
-  mx = warp_max(mx);
-  ss = warp_sum(ss);
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) { red[0][w] = mx; red[1][w] = ss; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = red[0][0], b = red[1][0];
-    for (int i = 1; i < NT / 64; ++i) { a = fmax(a, red[0][i]); b += red[1][i]; }
-    double* p = d.part + ((int64_t)site * gridDim.x + blockIdx.x) * 2;
-    p[0] = a;
-    p[1] = b;
-  }
-}
-
-// Sum the residual partials of each solute in a fixed order.
-// Grid: x = solute.  out[b*2 + {0,1}] = (max|r|, ||r||_2^2).
-__global__ void __launch_bounds__(NT)
-reduce_resid(const SoluteDesc* __restrict__ sd, int nparts, double* __restrict__ out) {
-  __shared__ double red[2][NT / 64];
-  const SoluteDesc d = sd[blockIdx.x];
-  if (!d.active) return;
-  double mx = 0.0, ss = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += NT) {
-    mx = fmax(mx, d.part[2 * i]);
-    ss += d.part[2 * i + 1];
-  }
-  mx = warp_max(mx);
-  ss = warp_sum(ss);
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) { red[0][w] = mx; red[1][w] = ss; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = red[0][0], b = red[1][0];
-    for (int i = 1; i < NT / 64; ++i) { a = fmax(a, red[0][i]); b += red[1][i]; }
-    // a NaN anywhere must surface: fmax drops NaNs, the sum keeps them
-    out[2 * blockIdx.x] = (b == b) ? a : b;
-    out[2 * blockIdx.x + 1] = b;
-  }
-}
-
-// ----------------------------------------------------------- MDIIS
-// dots[b][j] = <rnew, rh[j]> over m P elements.  Grid: x = chunks,
-// y = solute.  Partials to part[(b * gridDim.x + x) * MAX_DIIS + j].
-__global__ void __launch_bounds__(NT)
-diis_dots(const DiisDesc* __restrict__ dd, int64_t n, double* __restrict__ part) {
-  __shared__ double red[MAX_DIIS][NT / 64];
-  const DiisDesc d = dd[blockIdx.y];
-  if (!d.active) return;
-  double acc[MAX_DIIS];
+  double dot[MAX_DIIS];
 #pragma unroll
-  for (int j = 0; j < MAX_DIIS; ++j) acc[j] = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * NT;
-  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) {
-    const double a = d.rnew[e];
+  for (int j = 0; j < MAX_DIIS; ++j) dot[j] = 0.0;
+  for (int e = threadIdx.x; e < vrows * N0; e += NT) {
+    const T tv = t[e];
+    const T rv = zr[e] - tv;
+    r[e] = rv;
+    un[e] = tv + lam * rv;
+    const double rd = (double)rv;
+    mx = fmax(mx, fabs(rd));
+    ss += rd * rd;
 #pragma unroll
     for (int j = 0; j < MAX_DIIS; ++j)
-      if (j < d.nh) acc[j] += a * d.rh[j][e];
+      if (j < nd) dot[j] += rd * (double)static_cast<const T*>(d.rh[j])[off + e];
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  mx = warp_max(mx);
+  ss = warp_sum(ss);
+  if (lane == 0) { red[0][w] = mx; red[1][w] = ss; }
 #pragma unroll
   for (int j = 0; j < MAX_DIIS; ++j) {
-    if (j < d.nh) {
-      const double s = warp_sum(acc[j]);
-      if (lane == 0) red[j][w] = s;
+    if (j < nd) {
+      const double s = warp_sum(dot[j]);
+      if (lane == 0) red[2 + j][w] = s;
     }
   }
   __syncthreads();
-  if (threadIdx.x < d.nh) {
-    double s = red[threadIdx.x][0];
-    for (int i = 1; i < NT / 64; ++i) s += red[threadIdx.x][i];
-    part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * MAX_DIIS + threadIdx.x] = s;
+  if (threadIdx.x < 2 + nd) {
+    const int q = threadIdx.x;
+    double a = red[q][0];
+    for (int i = 1; i < NT / 64; ++i) a = (q == 0) ? fmax(a, red[q][i]) : a + red[q][i];
+    d.part[((int64_t)site * gridDim.x + blockIdx.x) * NPART + q] = a;
   }
 }
 
-// Fixed-order sum of the dot partials.  Grid: x = solute.
+// Fixed-order reduction of the residual-workgroup partials of each solute.
+// Grid: x = solute.  out[b * NPART + q]: max|r|, ||r||^2, <r, r_j>.
 __global__ void __launch_bounds__(NT)
-diis_dots_reduce(const DiisDesc* __restrict__ dd, int nchunks, const double* __restrict__ part,
-                 double* __restrict__ out) {
-  const DiisDesc d = dd[blockIdx.x];
+reduce_resid(const SoluteDesc* __restrict__ sd, int nparts, double* __restrict__ out) {
+  __shared__ double red[NPART][NT / 64];
+  const SoluteDesc& d = sd[blockIdx.x];
   if (!d.active) return;
-  const int j = threadIdx.x;
-  if (j < d.nh) {
-    double s = 0.0;
-    const double* p = part + (int64_t)blockIdx.x * nchunks * MAX_DIIS + j;
-    for (int c = 0; c < nchunks; ++c) s += p[(int64_t)c * MAX_DIIS];
-    out[blockIdx.x * MAX_DIIS + j] = s;
+  const int nq = 2 + d.ndot;
+  double acc[NPART];
+#pragma unroll
+  for (int q = 0; q < NPART; ++q) acc[q] = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += NT) {
+    const double* p = d.part + (int64_t)i * NPART;
+    acc[0] = fmax(acc[0], p[0]);
+#pragma unroll
+    for (int q = 1; q < NPART; ++q)
+      if (q < nq) acc[q] += p[q];
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  acc[0] = warp_max(acc[0]);
+  if (lane == 0) red[0][w] = acc[0];
+#pragma unroll
+  for (int q = 1; q < NPART; ++q) {
+    if (q < nq) {
+      const double s = warp_sum(acc[q]);
+      if (lane == 0) red[q][w] = s;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < nq) {
+    const int q = threadIdx.x;
+    double a = red[q][0];
+    for (int i = 1; i < NT / 64; ++i) a = (q == 0) ? fmax(a, red[q][i]) : a + red[q][i];
+    out[blockIdx.x * NPART + q] = a;  // fmax drops NaNs; the sums keep them
   }
 }
 
-// tout = sum_j coef_j (th_j + lam rh_j).  Grid: x = chunks, y = solute.
+// FP32 <-> FP64 copies of the iterate at a precision hand-over.
+// Grid: x = chunks, y = solute.
 __global__ void __launch_bounds__(NT)
-diis_update(const DiisDesc* __restrict__ dd, int64_t n) {
-  const DiisDesc d = dd[blockIdx.y];
-  if (!d.active) return;
+convert_batch(void* const* __restrict__ dst, const void* const* __restrict__ src, int to64,
+              int64_t n, const int* __restrict__ active) {
+  if (active && !active[blockIdx.y]) return;
   const int64_t stride = (int64_t)gridDim.x * NT;
   for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) {
-    double s = 0.0;
-#pragma unroll
-    for (int j = 0; j < MAX_DIIS; ++j)
-      if (j < d.nh) s += d.coef[j] * (d.th[j][e] + d.lam * d.rh[j][e]);
-    d.tout[e] = s;
+    if (to64)
+      static_cast<double*>(dst[blockIdx.y])[e] = (double)static_cast<const float*>(src[blockIdx.y])[e];
+    else
+      static_cast<float*>(dst[blockIdx.y])[e] = (float)static_cast<const double*>(src[blockIdx.y])[e];
   }
 }
 
@@ -581,17 +611,6 @@ mu_density(const SitesDesc* __restrict__ sites, GridArgs g, const double* const*
     for (int i = 1; i < NT / 64; ++i) s += red[threadIdx.x][i];
     part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 5 + threadIdx.x] = s;
   }
-}
-
-// Picard step t += lam r, or copy.  Grid: x = chunks, y = solute.
-__global__ void __launch_bounds__(NT)
-axpy_batch(double* const* __restrict__ y, const double* const* __restrict__ x, double a,
-           int64_t n, const int* __restrict__ active) {
-  if (active && !active[blockIdx.y]) return;
-  double* yy = y[blockIdx.y];
-  const double* xx = x[blockIdx.y];
-  const int64_t stride = (int64_t)gridDim.x * NT;
-  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) yy[e] += a * xx[e];
 }
 
 }  // namespace rism

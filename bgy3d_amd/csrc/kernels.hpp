@@ -12,18 +12,28 @@ constexpr int NT = 256;           // threads per workgroup, every kernel
 constexpr double EPSILON0INV = 331.84164;   // bgy3d.h:145
 constexpr double G_COULOMB = 1.2;           // bgy3d-solvents.h:9
 
-// Per-solute device pointers for one residual evaluation.  One entry per
-// solute of a batch; blockIdx.z selects the entry.
+// Per-solute device pointers for one MDIIS iteration (residual evaluation
+// fused with the DIIS combination and the Gram-matrix dot products).  One
+// entry per solute of a batch.  Vectors are [m][N2][N1][N0] in the working
+// type T of the launch (float in the FP32 phase, double in FP64).
 struct SoluteDesc {
-  const double* t;     // [m][N2][N1][N0] current iterate
-  double* r;           // [m][N2][N1][N0] residual out, F(t) = T(t) - t
-  const void* bv;      // [m][N2][N1][N0] beta*v_short, in the working type
-  void* W;             // [m][N2][N1][NH] spectral work, cpx<working type>
-  const void* Fs;      // [N2][N1][NH] form factor times s(k), cpx<working>
-  double* part;        // partial (max|r|, sum r^2) per residual workgroup
-  int active;          // 0: converged or failed, every workgroup returns
+  void* t;                   // iterate: written (ncomb > 0) or read (ncomb == 0)
+  const void* u[MAX_DIIS];   // u_j = t_j + lam r_j, the DIIS history
+  double coef[MAX_DIIS];     // t = sum_j coef_j u_j
+  int ncomb;                 // entries in the combination (0: t given)
+  int ndot;                  // history residuals to dot the new one with
+  const void* rh[MAX_DIIS];  // their residuals r_j
+  void* r;                   // new residual out, F(t) = T(t) - t
+  void* unew;                // new u = t + lam r out
+  double lam;
+  const void* bv;            // beta*v_short
+  void* W;                   // [m][N2][N1][NH] spectral work, cpx<T>
+  const void* Fs;            // [N2][N1][NH] form factor times s(k), cpx<T>
+  double* part;              // per residual workgroup: max|r|, sum r^2, dots
+  int active;                // 0: converged / failed / waiting: return
   int pad_;
 };
+constexpr int NPART = 2 + MAX_DIIS;   // doubles per residual workgroup
 
 // Grid and physics constants shared by a batch.
 struct GridArgs {
@@ -42,18 +52,6 @@ struct GridArgs {
   int closure;         // enum rism3d_closure
   int pse_order;       // n of PSE-n, 1 for KH
   double q[MAX_SITES]; // solvent charges
-};
-
-// MDIIS dot products and updates, one entry per solute.
-struct DiisDesc {
-  const double* rnew;            // newest residual
-  const double* rh[MAX_DIIS];    // residual history
-  const double* th[MAX_DIIS];    // iterate history
-  double coef[MAX_DIIS];         // DIIS coefficients
-  double* tout;                  // next iterate
-  double lam;                    // step applied to residuals
-  int nh;                        // history length
-  int active;
 };
 
 // Solute sites for the setup and mu kernels, per solute.

@@ -80,7 +80,7 @@ void launch_fft_y(int N1, dim3 grid, hipStream_t s, const SoluteDesc* sd, const 
 }
 template <typename T, int LINES>
 void launch_zstar_l(int N2, dim3 grid, size_t lds, hipStream_t s, const SoluteDesc* sd,
-                    const GridArgs& g, const cpx<T>* tw, const double* chi, const double* tau) {
+                    const GridArgs& g, const cpx<T>* tw, const T* chi, const T* tau) {
   switch (N2) {
 #define X(n) case n: hipLaunchKernelGGL((zstar<T, n, LINES>), grid, dim3(NT), lds, s, sd, g, tw, chi, tau); break;
     RISM_FOR_N(X)
@@ -95,10 +95,10 @@ int zstar_lines(int N2, int m) {
 }
 template <typename T>
 void launch_zstar(int N2, int NH, int N1, int B, hipStream_t s, const SoluteDesc* sd,
-                  const GridArgs& g, const cpx<T>* tw, const double* chi, const double* tau) {
+                  const GridArgs& g, const cpx<T>* tw, const T* chi, const T* tau) {
   const int L = zstar_lines<T>(N2, g.m);
   const size_t lds = (size_t)g.m * L * (N2 + 1) * sizeof(cpx<T>);
-  dim3 grid((NH + L - 1) / L, N1, B);
+  dim3 grid(B, (NH + L - 1) / L, N1);
   switch (L) {
     case 2: launch_zstar_l<T, 2>(N2, grid, lds, s, sd, g, tw, chi, tau); break;
     case 4: launch_zstar_l<T, 4>(N2, grid, lds, s, sd, g, tw, chi, tau); break;
@@ -206,26 +206,32 @@ bool diis_coefficients(const std::vector<double>& G, int nh, std::vector<double>
 
 // --------------------------------------------------------------- solver
 struct SoluteState {
-  // device memory, one slab each
-  DevBuf<double> th;       // D * mP  iterate history
-  DevBuf<double> rh;       // D * mP  residual history
-  DevBuf<double> bv64;     // mP
-  DevBuf<float> bv32;      // mP (mixed only)
-  DevBuf<cpx<double>> W;   // mK (fp64 view; fp32 view uses the front half)
+  // device memory
+  DevBuf<double> U;        // D slots of u_j = t_j + lam r_j, sized for FP64
+  DevBuf<double> R;        // D slots of r_j
+  DevBuf<double> t64;      // FP64 iterate
+  DevBuf<float> t32;       // FP32 iterate (mixed only)
+  DevBuf<double> bv64;     // beta v, FP64
+  DevBuf<float> bv32;      // and FP32 (mixed only)
+  DevBuf<cpx<double>> W;   // spectral work (FP32 phase uses the front half)
   DevBuf<cpx<double>> fs64;
   DevBuf<cpx<float>> fs32;
-  DevBuf<double> part;     // residual partials
+  DevBuf<double> part;     // residual-workgroup partials, NPART each
   DevBuf<double> sites;    // n * 6
   int n_sites = 0;
-  // host iteration state
-  std::vector<int> order;  // history slots, oldest first
-  std::vector<double> gram;  // D x D by slot
-  int cur = 0;             // slot of the iterate being evaluated
+  // host MDIIS state
+  std::vector<int> order;     // history slots, oldest first
+  std::vector<double> coef;   // t = sum coef_j u_{order_j} for the next step
+  std::vector<int> dotted;    // slots the pending residual is dotted with
+  int slot_new = 0;           // slot receiving the pending r and u
+  std::vector<double> gram;   // D x D Gram matrix by slot
   int active = 1;
   int status = RISM3D_NONCONVERGED;
   int iters = 0, iters32 = 0;
-  int phase32 = 0;         // 1 while the FP32 kernels run
-  int reached_mixed = 0;   // FP32 phase ended at mixed_tol
+  int phase32 = 0;            // 1 while the FP32 kernels run
+  int reached_mixed = 0;      // FP32 phase ended at mixed_tol or its floor
+  double best32 = INFINITY;
+  int stall32 = 0;
   double resid = INFINITY, best = INFINITY;
   int since_best = 0;
 };
@@ -234,20 +240,20 @@ struct Problem {
   GridArgs g;
   int N[3];
   double L[3], h[3];
-  int D;          // DIIS depth
+  int D;          // DIIS depth (1 = Picard)
   int mixed;
   int B;
   int64_t mP;
   hipStream_t s;
   Twiddles tw;
-  DevBuf<double> chi_tab, tau_tab;
+  DevBuf<double> chi_tab, tau_tab;      // 1D tables [pairs][nrad], [m][nrad]
+  DevBuf<double> chi3, tau3;            // on the 3D k grid, FP64
+  DevBuf<float> chi3f, tau3f;           // and FP32
   DevBuf<SoluteDesc> d_sd;
-  DevBuf<DiisDesc> d_dd;
   DevBuf<SitesDesc> d_sites;
-  DevBuf<double> d_out;       // B * 2 residual norms
-  DevBuf<double> d_dotpart;   // B * nchunks * MAX_DIIS
-  DevBuf<double> d_dots;      // B * MAX_DIIS
-  int nchunks = 0;
+  DevBuf<double> d_out;                 // B * NPART
+  DevBuf<void*> d_cdst, d_csrc;         // precision hand-over copies
+  DevBuf<int> d_cmask;
   int nparts = 0;
   std::vector<SoluteState> st;
 };
@@ -284,7 +290,7 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
     const dim3 gy((g.NH + LN - 1) / LN, g.m * g.N2, pb.B);
     mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p); mark(0, 1);
     mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(1, 1);
-    mark(2, 0); launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi_tab.p, pb.tau_tab.p); mark(2, 1);
+    mark(2, 0); launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
     mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(3, 1);
     mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p); mark(4, 1);
   } else {
@@ -293,28 +299,34 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
     const dim3 gy((g.NH + LN - 1) / LN, g.m * g.N2, pb.B);
     mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p); mark(0, 1);
     mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(1, 1);
-    mark(2, 0); launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi_tab.p, pb.tau_tab.p); mark(2, 1);
+    mark(2, 0); launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p); mark(2, 1);
     mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(3, 1);
     mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p); mark(4, 1);
   }
   hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, pb.d_sd.p, pb.nparts, pb.d_out.p);
   if (pf.on) {
-    int na = 0;
-    for (auto& d : sd) na += d.active != 0;
     const double sc = fp32 ? 4.0 : 8.0;  // working-type scalar
     const double mP = (double)g.m * g.P, mK = (double)g.m * g.K;
-    const double per[5] = {
-        mP * 8 + mP * sc + mK * 2 * sc,       // t, beta v in; C out
-        2 * mK * 2 * sc,                      // W in/out
-        2 * mK * 2 * sc + (double)g.K * 2 * sc,  // W in/out, F s in
-        2 * mK * 2 * sc,
-        mK * 2 * sc + mP * 8 + mP * 8};       // W, t in; r out
+    double b0 = 0, b4 = 0;
+    int na = 0;
+    for (auto& d : sd) {
+      if (!d.active) continue;
+      ++na;
+      // u_j in, t out (or t in), beta v in, C out
+      b0 += d.ncomb * mP * sc + mP * sc + mP * sc + mK * 2 * sc;
+      // W in, t in, r and u out, r_j in
+      b4 += mK * 2 * sc + mP * sc + 2 * mP * sc + d.ndot * mP * sc;
+    }
+    const double tables = (double)(g.m * (g.m + 1) / 2 + g.m) * g.K * sc;
+    const double per[5] = {b0, na * 2 * mK * 2 * sc,
+                           na * (2 * mK * 2 * sc + (double)g.K * 2 * sc) + tables,
+                           na * 2 * mK * 2 * sc, b4};
     HIPCHK(hipEventSynchronize(pf.ev[4][1]));
     for (int k = 0; k < 5; ++k) {
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, pf.ev[k][0], pf.ev[k][1]));
       pf.ms[k] += ms;
-      pf.bytes[k] += per[k] * na;
+      pf.bytes[k] += per[k];
       pf.launches[k] += 1;
     }
   }
@@ -325,24 +337,44 @@ struct Config {
   double lam, tol, mixed_tol;
 };
 
-// Runs the fixed-point iteration for every solute of the batch.
+// Copy the iterate between its FP32 and FP64 buffers for the solutes whose
+// mask entry is set.
+void convert_iterates(Problem& pb, const std::vector<int>& mask, bool to64) {
+  const int B = pb.B;
+  std::vector<void*> dst(B), src(B);
+  for (int b = 0; b < B; ++b) {
+    dst[b] = to64 ? (void*)pb.st[b].t64.p : (void*)pb.st[b].t32.p;
+    src[b] = to64 ? (void*)pb.st[b].t32.p : (void*)pb.st[b].t64.p;
+  }
+  HIPCHK(hipMemcpyAsync(pb.d_cdst.p, dst.data(), B * sizeof(void*), hipMemcpyHostToDevice, pb.s));
+  HIPCHK(hipMemcpyAsync(pb.d_csrc.p, src.data(), B * sizeof(void*), hipMemcpyHostToDevice, pb.s));
+  HIPCHK(hipMemcpyAsync(pb.d_cmask.p, mask.data(), B * sizeof(int), hipMemcpyHostToDevice, pb.s));
+  const int ch = (int)std::min<int64_t>(2048, (pb.mP + NT - 1) / NT);
+  hipLaunchKernelGGL(convert_batch, dim3(ch, B), dim3(NT), 0, pb.s, pb.d_cdst.p,
+                     (const void* const*)pb.d_csrc.p, to64 ? 1 : 0, pb.mP, pb.d_cmask.p);
+  HIPCHK(hipStreamSynchronize(pb.s));
+}
+
+// Runs the fixed-point iteration for every solute of the batch.  MDIIS
+// (Pulay) over the history u_j = t_j + lam r_j: t = sum c_j u_j with c
+// minimising |sum c_j r_j|, sum c_j = 1.  Depth 1 is Picard, t += lam r
+// (bgy3d_snes_picard, bgy3d-snes.c:348-384).  One host synchronisation per
+// iteration, for the norms and the Gram row.
 void iterate_batch(Problem& pb, const Config& cf) {
   const int B = pb.B;
   const int64_t mP = pb.mP;
   const int D = pb.D;
   std::vector<SoluteDesc> sd(B);
-  std::vector<DiisDesc> dd(B);
-  std::vector<double> out(B * 2), dots(B * MAX_DIIS);
-  const bool mdiis = cf.solver == RISM3D_SOLVER_MDIIS;
+  std::vector<double> out((size_t)B * NPART);
   for (int b = 0; b < B; ++b) {
     auto& s = pb.st[b];
     s.phase32 = cf.precision == RISM3D_MIXED;
     s.gram.assign(D * D, 0.0);
     s.order.clear();
-    s.cur = 0;
+    s.coef.clear();
   }
+  if (cf.precision == RISM3D_MIXED) convert_iterates(pb, std::vector<int>(B, 1), false);
   int polish_left = cf.polish;
-  const dim3 gchunk(pb.nchunks, B);
   for (;;) {
     bool any = false, any32 = false;
     for (int b = 0; b < B; ++b) {
@@ -350,74 +382,82 @@ void iterate_batch(Problem& pb, const Config& cf) {
       if (s.active) { any = true; any32 |= (s.phase32 != 0); }
     }
     if (!any) break;
-    // All active solutes of a launch run one precision: FP32 while any
-    // still needs FP32 (the others wait), then FP64.
+    // All solutes of a launch run one precision: FP32 while any still needs
+    // FP32 (the others wait), then FP64.
     const bool fp32 = any32;
+    const size_t sz = fp32 ? 4 : 8;
     for (int b = 0; b < B; ++b) {
       auto& s = pb.st[b];
-      const bool run = s.active && (s.phase32 != 0) == fp32;
-      sd[b].t = s.th.p + (int64_t)s.cur * mP;
-      sd[b].r = s.rh.p + (int64_t)s.cur * mP;
-      sd[b].bv = fp32 ? (const void*)s.bv32.p : (const void*)s.bv64.p;
-      sd[b].W = s.W.p;
-      sd[b].Fs = fp32 ? (const void*)s.fs32.p : (const void*)s.fs64.p;
-      sd[b].part = s.part.p;
-      sd[b].active = run;
-      // history including the slot being evaluated now
-      dd[b] = DiisDesc{};
-      dd[b].active = run && mdiis;
-      if (run && mdiis) {
-        std::vector<int> hist = s.order;
-        hist.erase(std::remove(hist.begin(), hist.end(), s.cur), hist.end());
-        hist.push_back(s.cur);
-        if ((int)hist.size() > D) hist.erase(hist.begin());
-        dd[b].rnew = s.rh.p + (int64_t)s.cur * mP;
-        dd[b].nh = (int)hist.size();
-        for (int j = 0; j < dd[b].nh; ++j) {
-          dd[b].rh[j] = s.rh.p + (int64_t)hist[j] * mP;
-          dd[b].th[j] = s.th.p + (int64_t)hist[j] * mP;
-        }
+      SoluteDesc& d = sd[b];
+      std::memset(&d, 0, sizeof d);
+      d.active = s.active && (s.phase32 != 0) == fp32;
+      if (!d.active) continue;
+      auto slotU = [&](int q) { return (void*)((char*)s.U.p + (size_t)q * mP * sz); };
+      auto slotR = [&](int q) { return (void*)((char*)s.R.p + (size_t)q * mP * sz); };
+      d.t = fp32 ? (void*)s.t32.p : (void*)s.t64.p;
+      d.ncomb = (int)s.coef.size();
+      for (int j = 0; j < d.ncomb; ++j) {
+        d.u[j] = slotU(s.order[j]);
+        d.coef[j] = s.coef[j];
       }
+      // slot for the new r, u: a free one, else the oldest (dropped)
+      s.dotted = s.order;
+      if ((int)s.order.size() < D) {
+        int q = 0;
+        while (std::find(s.order.begin(), s.order.end(), q) != s.order.end()) ++q;
+        s.slot_new = q;
+      } else {
+        s.slot_new = s.order.front();
+        s.dotted.erase(s.dotted.begin());
+      }
+      d.ndot = (int)s.dotted.size();
+      for (int j = 0; j < d.ndot; ++j) d.rh[j] = slotR(s.dotted[j]);
+      d.r = slotR(s.slot_new);
+      d.unew = slotU(s.slot_new);
+      d.lam = cf.lam;
+      d.bv = fp32 ? (const void*)s.bv32.p : (const void*)s.bv64.p;
+      d.W = s.W.p;
+      d.Fs = fp32 ? (const void*)s.fs32.p : (const void*)s.fs64.p;
+      d.part = s.part.p;
     }
     evaluate(pb, fp32, sd);
-    if (mdiis) {
-      HIPCHK(hipMemcpyAsync(pb.d_dd.p, dd.data(), B * sizeof(DiisDesc), hipMemcpyHostToDevice, pb.s));
-      hipLaunchKernelGGL(diis_dots, gchunk, dim3(NT), 0, pb.s, pb.d_dd.p, mP, pb.d_dotpart.p);
-      hipLaunchKernelGGL(diis_dots_reduce, dim3(B), dim3(NT), 0, pb.s, pb.d_dd.p, pb.nchunks,
-                         pb.d_dotpart.p, pb.d_dots.p);
-      HIPCHK(hipMemcpyAsync(dots.data(), pb.d_dots.p, B * MAX_DIIS * sizeof(double), hipMemcpyDeviceToHost, pb.s));
-    }
-    HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, B * 2 * sizeof(double), hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, pb.s));
     HIPCHK(hipStreamSynchronize(pb.s));
     HIPCHK(hipGetLastError());
 
-    bool any_update = false;
+    std::vector<int> handover(B, 0);
+    bool any_handover = false;
     for (int b = 0; b < B; ++b) {
       auto& s = pb.st[b];
-      if (!sd[b].active) { dd[b].active = 0; continue; }
-      const double rinf = out[2 * b];
+      if (!sd[b].active) continue;
+      const double* o = out.data() + (size_t)b * NPART;
+      const double rinf = o[0], r2 = o[1];
       s.iters++;
       if (fp32) s.iters32++;
       s.resid = rinf;
       if (cf.verbose)
         std::fprintf(stderr, "rism3d: solute %d iter %d %s |F|inf = %.3e |F|2 = %.3e\n", b, s.iters,
-                     fp32 ? "fp32" : "fp64", rinf, std::sqrt(out[2 * b + 1]));
-      if (!std::isfinite(rinf)) {
+                     fp32 ? "fp32" : "fp64", rinf, std::sqrt(r2));
+      if (!std::isfinite(rinf) || !std::isfinite(r2)) {
         s.status = RISM3D_NAN;
         s.active = 0;
-        dd[b].active = 0;
         continue;
       }
       if (fp32) {
-        if (rinf <= cf.mixed_tol || s.iters32 >= cf.max_iter) {
-          // hand over to FP64: restart the history at this iterate
-          s.reached_mixed = rinf <= cf.mixed_tol;
+        // FP32 hands over at mixed_tol, or once its residual stops falling
+        // (the FP32 floor, ~1e-7 max|t|, can lie above mixed_tol near
+        // charged sites): no 30% gain on the best value in 12 iterations.
+        if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; } else { s.stall32++; }
+        if (rinf <= cf.mixed_tol || s.stall32 >= 12 || s.iters32 >= cf.max_iter) {
+          s.reached_mixed = rinf <= cf.mixed_tol || s.stall32 >= 12;
           s.phase32 = 0;
           s.order.clear();
-          dd[b].active = 0;
+          s.coef.clear();
           s.best = INFINITY;
           s.since_best = 0;
-          continue;  // the same slot is re-evaluated in FP64
+          handover[b] = 1;
+          any_handover = true;
+          continue;  // the same iterate is evaluated again in FP64
         }
       } else {
         const bool polish_done = cf.precision == RISM3D_MIXED && polish_left <= 0;
@@ -425,81 +465,47 @@ void iterate_batch(Problem& pb, const Config& cf) {
           s.status = (rinf <= cf.tol || (cf.precision == RISM3D_MIXED && s.reached_mixed))
                          ? RISM3D_OK : RISM3D_NONCONVERGED;
           s.active = 0;
-          dd[b].active = 0;
           continue;
         }
         if (s.iters - s.iters32 >= cf.max_iter) {
           s.status = RISM3D_NONCONVERGED;
           s.active = 0;
-          dd[b].active = 0;
           continue;
         }
       }
-      if (!mdiis) {
-        // Picard: t_next = t + lam r, in a fresh slot (bgy3d-snes.c:367-383)
-        const int next = (s.cur + 1) % D;
-        dd[b].active = 1;
-        dd[b].nh = 1;
-        dd[b].rh[0] = s.rh.p + (int64_t)s.cur * mP;
-        dd[b].th[0] = s.th.p + (int64_t)s.cur * mP;
-        dd[b].coef[0] = 1.0;
-        dd[b].lam = cf.lam;
-        dd[b].tout = s.th.p + (int64_t)next * mP;
-        s.cur = next;
-        any_update = true;
-        continue;
+      // enter the new entry, refresh its Gram row
+      s.order = s.dotted;
+      s.order.push_back(s.slot_new);
+      for (int j = 0; j < (int)s.dotted.size(); ++j) {
+        s.gram[s.slot_new * D + s.dotted[j]] = o[2 + j];
+        s.gram[s.dotted[j] * D + s.slot_new] = o[2 + j];
       }
-      // MDIIS bookkeeping: enter the new slot, refresh its Gram row.
-      std::vector<int> hist = s.order;
-      hist.erase(std::remove(hist.begin(), hist.end(), s.cur), hist.end());
-      hist.push_back(s.cur);
-      if ((int)hist.size() > D) hist.erase(hist.begin());
-      for (int j = 0; j < (int)hist.size(); ++j) {
-        s.gram[s.cur * D + hist[j]] = dots[b * MAX_DIIS + j];
-        s.gram[hist[j] * D + s.cur] = dots[b * MAX_DIIS + j];
-      }
-      // restart when the residual blows up against the best one seen
-      const double r2 = out[2 * b + 1];
+      s.gram[s.slot_new * D + s.slot_new] = r2;
+      // restart when the residual blows up against the best one, or stalls
       if (r2 < s.best) { s.best = r2; s.since_best = 0; } else { s.since_best++; }
       if (r2 > 100.0 * s.best || s.since_best > 3 * D) {
-        hist.assign(1, s.cur);
+        s.order.assign(1, s.slot_new);
         s.best = r2;
         s.since_best = 0;
       }
-      s.order = hist;
-      const int nh = (int)hist.size();
-      std::vector<double> G(nh * nh), c;
+      const int nh = (int)s.order.size();
+      std::vector<double> G(nh * nh);
       for (int i = 0; i < nh; ++i)
-        for (int j = 0; j < nh; ++j) G[i * nh + j] = s.gram[hist[i] * D + hist[j]];
-      if (!diis_coefficients(G, nh, c)) {
-        s.order.assign(1, s.cur);
-        c.assign(1, 1.0);
+        for (int j = 0; j < nh; ++j) G[i * nh + j] = s.gram[s.order[i] * D + s.order[j]];
+      if (nh == 1 || !diis_coefficients(G, nh, s.coef)) {
+        s.order.assign(1, s.slot_new);
+        s.coef.assign(1, 1.0);
       }
-      const int nh2 = (int)s.order.size();
-      // next slot: one not in the history, else the oldest
-      int next = -1;
-      for (int q = 0; q < D && next < 0; ++q)
-        if (std::find(s.order.begin(), s.order.end(), q) == s.order.end()) next = q;
-      if (next < 0) next = s.order.front();
-      dd[b].active = 1;
-      dd[b].nh = nh2;
-      for (int j = 0; j < nh2; ++j) {
-        dd[b].rh[j] = s.rh.p + (int64_t)s.order[j] * mP;
-        dd[b].th[j] = s.th.p + (int64_t)s.order[j] * mP;
-        dd[b].coef[j] = c[j];
-      }
-      dd[b].lam = cf.lam;
-      dd[b].tout = s.th.p + (int64_t)next * mP;
-      if (next == s.order.front() && nh2 == D) s.order.erase(s.order.begin());
-      s.cur = next;
-      any_update = true;
     }
+    if (any_handover) convert_iterates(pb, handover, true);
     if (!fp32 && cf.precision == RISM3D_MIXED) polish_left--;
-    if (any_update) {
-      HIPCHK(hipMemcpyAsync(pb.d_dd.p, dd.data(), B * sizeof(DiisDesc), hipMemcpyHostToDevice, pb.s));
-      hipLaunchKernelGGL(diis_update, gchunk, dim3(NT), 0, pb.s, pb.d_dd.p, mP);
-    }
   }
+  // solutes that stopped inside the FP32 phase (NaN, limits): FP64 copy
+  std::vector<int> mask(B, 0);
+  bool any32 = false;
+  for (int b = 0; b < B; ++b)
+    if (pb.st[b].phase32) { mask[b] = 1; any32 = true; }
+  if (any32) convert_iterates(pb, mask, true);
 }
 
 struct Inputs {
@@ -564,7 +570,7 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   const rism3d_problem& pd = *in.pd;
   Problem pb;
   pb.B = in.B;
-  pb.D = in.opt.solver == RISM3D_SOLVER_PICARD ? 2 : in.opt.diis_depth;
+  pb.D = in.opt.solver == RISM3D_SOLVER_PICARD ? 1 : in.opt.diis_depth;
   pb.mixed = in.opt.precision == RISM3D_MIXED;
   for (int d = 0; d < 3; ++d) { pb.N[d] = pd.N[d]; pb.L[d] = pd.L[d]; pb.h[d] = pd.L[d] / pd.N[d]; }
   GridArgs& g = pb.g;
@@ -615,6 +621,16 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   pb.tau_tab = DevBuf<double>(tau_tab.size());
   HIPCHK(hipMemcpyAsync(pb.chi_tab.p, chi_tab.data(), chi_tab.size() * 8, hipMemcpyHostToDevice, pb.s));
   HIPCHK(hipMemcpyAsync(pb.tau_tab.p, tau_tab.data(), tau_tab.size() * 8, hipMemcpyHostToDevice, pb.s));
+  {
+    const size_t np = (size_t)m * (m + 1) / 2;
+    pb.chi3 = DevBuf<double>(np * g.K);
+    pb.chi3f = DevBuf<float>(np * g.K);
+    pb.tau3 = DevBuf<double>((size_t)m * g.K);
+    pb.tau3f = DevBuf<float>((size_t)m * g.K);
+    const int ck = (int)std::min<int64_t>(4096, (g.K + NT - 1) / NT);
+    hipLaunchKernelGGL(ktab3, dim3(ck), dim3(NT), 0, pb.s, g, pb.chi_tab.p, pb.tau_tab.p,
+                       pb.chi3.p, pb.chi3f.p, pb.tau3.p, pb.tau3f.p);
+  }
 
   // per-solute buffers
   const int B = in.B;
@@ -629,14 +645,16 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   std::vector<std::vector<double>> site_host(B);
   for (int b = 0; b < B; ++b) {
     auto& s = pb.st[b];
-    s.th = DevBuf<double>((size_t)pb.D * pb.mP);
-    s.rh = DevBuf<double>((size_t)pb.D * pb.mP);
+    s.U = DevBuf<double>((size_t)pb.D * pb.mP);
+    s.R = DevBuf<double>((size_t)pb.D * pb.mP);
+    s.t64 = DevBuf<double>(pb.mP);
+    if (pb.mixed) s.t32 = DevBuf<float>(pb.mP);
     s.bv64 = DevBuf<double>(pb.mP);
     if (pb.mixed) s.bv32 = DevBuf<float>(pb.mP);
     s.W = DevBuf<cpx<double>>(mK);
     s.fs64 = DevBuf<cpx<double>>(g.K);
     if (pb.mixed) s.fs32 = DevBuf<cpx<float>>(g.K);
-    s.part = DevBuf<double>((size_t)pb.nparts * 2);
+    s.part = DevBuf<double>((size_t)pb.nparts * NPART);
     s.n_sites = in.n_sites[b];
     auto& sh = site_host[b];
     sh.resize((size_t)s.n_sites * 6);
@@ -655,17 +673,16 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     fs32p[b] = pb.mixed ? s.fs32.p : nullptr;
     // initial iterate: zero (hnc3d.c:1845) or the restart token
     if (t_init && b == 0)
-      HIPCHK(hipMemcpyAsync(s.th.p, t_init->data(), pb.mP * 8, hipMemcpyHostToDevice, pb.s));
+      HIPCHK(hipMemcpyAsync(s.t64.p, t_init->data(), pb.mP * 8, hipMemcpyHostToDevice, pb.s));
     else
-      HIPCHK(hipMemsetAsync(s.th.p, 0, pb.mP * 8, pb.s));
+      HIPCHK(hipMemsetAsync(s.t64.p, 0, pb.mP * 8, pb.s));
   }
   pb.d_sd = DevBuf<SoluteDesc>(B);
-  pb.d_dd = DevBuf<DiisDesc>(B);
   pb.d_sites = DevBuf<SitesDesc>(B);
-  pb.d_out = DevBuf<double>(2 * B);
-  pb.nchunks = (int)std::min<int64_t>(1024, (pb.mP + NT * 8 - 1) / (NT * 8));
-  pb.d_dotpart = DevBuf<double>((size_t)B * pb.nchunks * MAX_DIIS);
-  pb.d_dots = DevBuf<double>((size_t)B * MAX_DIIS);
+  pb.d_out = DevBuf<double>((size_t)B * NPART);
+  pb.d_cdst = DevBuf<void*>(B);
+  pb.d_csrc = DevBuf<void*>(B);
+  pb.d_cmask = DevBuf<int>(B);
   HIPCHK(hipMemcpyAsync(pb.d_sites.p, sites.data(), B * sizeof(SitesDesc), hipMemcpyHostToDevice, pb.s));
 
   // setup: beta v (field0) and F(k) s(k)
@@ -700,17 +717,27 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   if (in.t_eval) {
     // one residual evaluation F(t) = T(t) - t (iterate_t1, hnc3d.c:1184)
     auto& s0 = pb.st[0];
-    HIPCHK(hipMemcpyAsync(s0.th.p, in.t_eval, pb.mP * 8, hipMemcpyHostToDevice, pb.s));
+    HIPCHK(hipMemcpyAsync(s0.t64.p, in.t_eval, pb.mP * 8, hipMemcpyHostToDevice, pb.s));
+    if (pb.mixed) convert_iterates(pb, std::vector<int>(1, 1), false);
     std::vector<SoluteDesc> sd(1);
-    sd[0].t = s0.th.p;
-    sd[0].r = s0.rh.p;
+    std::memset(sd.data(), 0, sizeof(SoluteDesc));
+    sd[0].t = pb.mixed ? (void*)s0.t32.p : (void*)s0.t64.p;
+    sd[0].r = s0.R.p;
+    sd[0].unew = s0.U.p;
     sd[0].bv = pb.mixed ? (const void*)s0.bv32.p : (const void*)s0.bv64.p;
     sd[0].W = s0.W.p;
     sd[0].Fs = pb.mixed ? (const void*)s0.fs32.p : (const void*)s0.fs64.p;
     sd[0].part = s0.part.p;
     sd[0].active = 1;
     evaluate(pb, pb.mixed != 0, sd);
-    HIPCHK(hipMemcpyAsync(in.r_out, s0.rh.p, pb.mP * 8, hipMemcpyDeviceToHost, pb.s));
+    if (pb.mixed) {
+      // FP32 residual to FP64 through the (now free) iterate buffers
+      HIPCHK(hipMemcpyAsync(s0.t32.p, s0.R.p, pb.mP * 4, hipMemcpyDeviceToDevice, pb.s));
+      convert_iterates(pb, std::vector<int>(1, 1), true);
+      HIPCHK(hipMemcpyAsync(in.r_out, s0.t64.p, pb.mP * 8, hipMemcpyDeviceToHost, pb.s));
+    } else {
+      HIPCHK(hipMemcpyAsync(in.r_out, s0.R.p, pb.mP * 8, hipMemcpyDeviceToHost, pb.s));
+    }
     HIPCHK(hipStreamSynchronize(pb.s));
     HIPCHK(hipGetLastError());
     res[0].status = RISM3D_OK;
@@ -736,7 +763,7 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     std::vector<DevBuf<double>> gbuf(B);
     bool want_g = false;
     for (int b = 0; b < B; ++b) {
-      tt[b] = pb.st[b].th.p + (int64_t)pb.st[b].cur * pb.mP;
+      tt[b] = pb.st[b].t64.p;
       if (res[b].g) {
         gbuf[b] = DevBuf<double>(pb.mP);
         gg[b] = gbuf[b].p;
