@@ -210,3 +210,17 @@ def test_batch_reports_per_solute_nan(chi_w):
     print(f"  statuses {[o['status'] for o in out]}", flush=True)
     assert out[0]["status"] == "OK"
     assert out[1]["status"] in ("OK", "NAN")
+
+
+def test_readme_energy_hnc_end_to_end():
+    """README:20-28 as the reference runs it: chi is not supplied, so the
+    solver builds it with 1D DRISM (eps 78.4) on the upscaled grid
+    rmax = 40, nrad = 1536 (hnc3d.c:772-783, 1471-1495), then solves."""
+    print("case readme-96-chi-null", flush=True)
+    R = _R()
+    s = {"N": 96, "L": 10.0, "rho": 0.0333295, "beta": 1.6889, "dielectric": 78.4,
+         "norm-tol": 1e-12, "max-iter": 2000}
+    out = R.hnc3d_run_solute(W, W, s, chi=None)
+    print(f"  mu {out['free-energy']:.10f} README -6.349052", flush=True)
+    assert abs(out["free-energy"] - (-6.349052)) < 1e-6
+    assert abs(out["free-energy"] - (-6.349051635500067)) < 1e-8
