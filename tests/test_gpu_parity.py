@@ -224,3 +224,17 @@ def test_readme_energy_hnc_end_to_end():
     print(f"  mu {out['free-energy']:.10f} README -6.349052", flush=True)
     assert abs(out["free-energy"] - (-6.349052)) < 1e-6
     assert abs(out["free-energy"] - (-6.349051635500067)) < 1e-8
+
+
+def test_cli_energy_hnc(capsys):
+    """The README command through the energy --hnc equivalent."""
+    print("case cli-energy-hnc", flush=True)
+    from bgy3d_amd import cli
+    rc = cli.main(["energy", "--hnc", "--norm-tol", "1e-12", "--dielectric", "78.4",
+                   "--rho", "0.0333295", "--beta", "1.6889", "--L", "10", "--N", "96",
+                   "--solvent", "water, PR-SPC/E", "--solute", "water, PR-SPC/E",
+                   "--chi-upscale"])
+    out = capsys.readouterr().out.strip().splitlines()[-1]
+    print(f"  printed {out}", flush=True)
+    assert rc == 0
+    assert abs(float(out) - (-6.349052)) < 1e-6
