@@ -105,7 +105,7 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
       } else {
         tv = t[e];
       }
-      c = closure_c<T>(g.closure, g.pse_order, bv[e], tv);
+      c = (d.mode == 0) ? closure_c<T>(g.closure, g.pse_order, bv[e], tv) : tv;
     }
     ldsr[e] = c;  // z_n = c_{2n} + i c_{2n+1}, row stride M complex
   }
@@ -245,6 +245,18 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   }
   __syncthreads();
   for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, false>(lds + s * LINES * LS, LS, 1, tw);
+  if (d.mode == 3) {  // forward spectrum only
+    for (int s = 0; s < m; ++s) {
+      cpx<T>* Ws = W + s * g.K;
+      const cpx<T>* L = lds + s * LINES * LS;
+      for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
+        const int z = e / LINES, l = e - z * LINES;
+        if (kx0 + l < g.NH) Ws[z * zs + l] = L[l * LS + z];
+      }
+    }
+    return;
+  }
+  const bool linear = d.mode != 0;  // chi convolution only
 
   const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + base;
   const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
@@ -257,7 +269,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     cpx<T> c[MAX_SITES];
     for (int s = 0; s < m; ++s) c[s] = lds[(s * LINES + l) * LS + z];
     cpx<T> uc = {T(0), T(0)};
-    if (!g.renorm) {
+    if (!g.renorm && !linear) {
       // u_c(k) = 4 pi eps0^-1 exp(-k^2/4G^2)/k^2 F(k) s(k), 0 at k = 0
       // (bgy3d-force.h:140-147, bgy3d-solutes.c:767-792).
       const int iz = (z <= N2 / 2) ? z : z - N2;
@@ -279,7 +291,9 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
         acc.im += chi * c[j].im;
       }
       cpx<T> out;
-      if (g.renorm) {
+      if (linear) {
+        out = cscale(acc, T(g.a_scale));
+      } else if (g.renorm) {
         // (h^3/L^3) sum_j chi_ij C_j - (beta/eps0/L^3) tau_i F s  (hnc3d.c:1245-1268)
         const T tau = tau3[i * g.K + kk] * T(g.tau_scale);
         out = csub(cscale(acc, T(g.a_scale)), cscale(fs, tau));
@@ -361,9 +375,13 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   double dot[MAX_DIIS];
 #pragma unroll
   for (int j = 0; j < MAX_DIIS; ++j) dot[j] = 0.0;
+  const int mode = d.mode;
   for (int e = threadIdx.x; e < vrows * N0; e += NT) {
     const T tv = t[e];
-    const T rv = zr[e] - tv;
+    T rv;
+    if (mode == 0) rv = zr[e] - tv;                                    // F(t)
+    else if (mode == 1) rv = T(d.dt[off + e]) * zr[e] + T(d.gv[off + e]) - tv;  // adjoint
+    else rv = zr[e];                                                   // K t
     r[e] = rv;
     un[e] = tv + lam * rv;
     const double rd = (double)rv;
@@ -610,6 +628,237 @@ mu_density(const SitesDesc* __restrict__ sites, GridArgs g, const double* const*
     double s = red[threadIdx.x][0];
     for (int i = 1; i < NT / 64; ++i) s += red[threadIdx.x][i];
     part[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 5 + threadIdx.x] = s;
+  }
+}
+
+// ------------------------------------------------------------ gradients
+// Derivatives of the capped pair functions (bgy3d-solutes.c:102-131,
+// 162-190, 246-305).
+__device__ __forceinline__ double ljcap1(double r) {
+  const double R = 0.2;
+  if (r >= R) return lj1(r);
+  return 2 * (lj1(R) / (2 * R)) * r;
+}
+__device__ __forceinline__ double cscap1(double r) {
+  const double R = 0.2;
+  if (r >= R) return cs1(r);
+  return 2 * (cs1(R) / (2 * R)) * r;
+}
+__device__ __forceinline__ double cl1(double r) {
+  const double r2 = r * r;
+  if (r >= 0.02) return ((2 / sqrt(M_PI)) * r * exp(-r2) - erf(r)) / r2;
+  return -(4 / (3 * sqrt(M_PI))) * r * (1. + r2 * (-3. / 5. + r2 * (3. / 14. + r2 * (-1. / 18.))));
+}
+__device__ __forceinline__ double ccap1(double r) {
+  if (r >= 0.2) return -1.0 / (r * r);
+  return cscap1(r) + cl1(r);
+}
+
+// d c / d t and d c / d(beta v) of closure_c, FP64.
+__device__ __forceinline__ void closure_derivs(int method, int order, double bv, double t,
+                                               double* ct, double* cv) {
+  const double x = -bv + t;
+  switch (method) {
+    case 0: *ct = expm1(x); *cv = -(*ct + 1); return;
+    case 1:
+      if (x <= 0) { *ct = expm1(x); *cv = -(*ct + 1); } else { *ct = 0; *cv = -1; }
+      return;
+    case 2: *ct = exp(-bv) - 1; *cv = -exp(-bv) * (1 + t); return;
+    default: {
+      *ct = (x < 0) ? expm1(x) : exps1(order - 1, x);
+      *cv = -(*ct + 1);
+    }
+  }
+}
+
+// form1 (closures.f90:615-653): differential of the mu density.
+__device__ __forceinline__ double form1_mu(int method, int order, double x, double dx, double h,
+                                           double dh, double c, double dc, double cl) {
+  const double base = -dc - dh * (c + cl) / 2 - h * dc / 2;
+  switch (method) {
+    case 0: return base + h * dh;
+    case 1: return (h < 0) ? base + h * dh : base;
+    case 2: return base;
+    default: {
+      if (h < 0) return base + h * dh;
+      double nx = 1, xn = 1;
+      for (int k = 1; k <= order; ++k) { nx *= k; xn *= x; }
+      return base + (h * dh - xn * dx / nx);
+    }
+  }
+}
+
+// Per-point coefficients at the converged t (grid-stride, one solute).
+// mode 0 (TPT, hnc3d.c:1666-1742): A_i = rho h^3 g_i, B = 0.
+// mode 1 (adjoint, prep): D_i = dc/dt, gmu_i = dmu/dt.
+// mode 2 (adjoint, final): A_i = beta (K lambda)_i dc/d(beta v) + dmu/dv_i,
+//                          B = rho h^3 / 2 sum_i q_i h_i  (dmu/du_c).
+__global__ void __launch_bounds__(NT)
+adj_coef(GridArgs g, SitesDesc sd, const double* __restrict__ t, const double* __restrict__ bv,
+         const double* __restrict__ Klam, double L0, double L1, double L2, double h0, double h1,
+         double h2, double rho, int mode, double* __restrict__ A, double* __restrict__ B,
+         double* __restrict__ D, double* __restrict__ gmu) {
+  const double fmu = rho * g.h3 / g.beta;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.P; e += stride) {
+    double uc = 0.0;
+    if (mode != 0) {
+      const int x = (int)(e % g.N0);
+      const int y = (int)((e / g.N0) % g.N1);
+      const int z = (int)(e / ((int64_t)g.N0 * g.N1));
+      const double rx = x * h0 - L0 / 2, ry = y * h1 - L1 / 2, rz = z * h2 - L2 / 2;
+      for (int a = 0; a < sd.n; ++a) {
+        const double* s = sd.xyz_s_e_q + 6 * a;
+        const double dx = rx - s[0], dy = ry - s[1], dz = rz - s[2];
+        uc += s[5] * G_COULOMB * cl0(G_COULOMB * sqrt(dx * dx + dy * dy + dz * dz));
+      }
+      uc *= EPSILON0INV;
+    }
+    double bsum = 0.0;
+    for (int i = 0; i < g.m; ++i) {
+      const int64_t q = i * g.P + e;
+      const double ti = t[q], bvi = bv[q];
+      const double c = closure_c<double>(g.closure, g.pse_order, bvi, ti);
+      const double h = c + ti;
+      if (mode == 0) {
+        A[q] = rho * g.h3 * (h + 1.0);
+        continue;
+      }
+      double ct, cv;
+      closure_derivs(g.closure, g.pse_order, bvi, ti, &ct, &cv);
+      const double x = -bvi + ti;
+      const double cl = -g.beta * g.q[i] * uc;
+      if (mode == 1) {
+        D[q] = ct;
+        gmu[q] = fmu * form1_mu(g.closure, g.pse_order, x, 1.0, h, ct + 1, c, ct, cl);
+      } else {
+        const double dcv = cv * g.beta;
+        A[q] = g.beta * Klam[q] * cv +
+               fmu * form1_mu(g.closure, g.pse_order, x, -g.beta, h, dcv, c, dcv, cl);
+        bsum += g.q[i] * h;
+      }
+    }
+    if (mode == 2) B[e] = 0.5 * rho * g.h3 * bsum;
+  }
+}
+
+// sum_r sum_i A_i(r) dv_i(r)/dtheta_a (+ B(r) du_c/dtheta_a) for solute
+// site a = blockIdx.y over a chunk of points blockIdx.x.  mode 0 (TPT,
+// field1 bgy3d-solutes.c:381-468): full capped Coulomb, xyz only.
+// mode 1 (adjoint): short-range v plus the u_c of the mu integral; x, y, z,
+// q, sigma, epsilon.  Partials part[(a * gridDim.x + chunk) * 6 + p].
+__global__ void __launch_bounds__(NT)
+grad_field(GridArgs g, SitesDesc sd, const double* __restrict__ solvent_sse,
+           const double* __restrict__ A, const double* __restrict__ B, double L0, double L1,
+           double L2, double h0, double h1, double h2, int mode, double* __restrict__ part) {
+  __shared__ double red[6][NT / 64];
+  const int a = blockIdx.y;
+  const double* s = sd.xyz_s_e_q + 6 * a;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  const double G = G_COULOMB;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.P; e += stride) {
+    const int x = (int)(e % g.N0);
+    const int y = (int)((e / g.N0) % g.N1);
+    const int z = (int)(e / ((int64_t)g.N0 * g.N1));
+    const double dx = x * h0 - L0 / 2 - s[0], dy = y * h1 - L1 / 2 - s[1], dz = z * h2 - L2 / 2 - s[2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double inv = r > 0 ? 1.0 / r : 0.0;
+    const double gx = -dx * inv, gy = -dy * inv, gz = -dz * inv;  // dr / dx_a
+    double fr = 0.0;
+    for (int i = 0; i < g.m; ++i) {
+      const double* sv = solvent_sse + 3 * i;
+      const double eab = sqrt(sv[1] * s[4]);
+      const double sab = 0.5 * (sv[0] + s[3]);
+      const double qab = sv[2] * s[5] * EPSILON0INV;
+      const double Ai = A[i * g.P + e];
+      const double rs = r / sab;
+      const double lj1v = ljcap1(rs);
+      if (mode == 0) {
+        fr += Ai * ((eab / sab) * lj1v + qab * G * G * ccap1(G * r));
+      } else {
+        fr += Ai * ((eab / sab) * lj1v + qab * G * G * cscap1(G * r));
+        acc[3] += Ai * sv[2] * EPSILON0INV * G * cscap0(G * r);
+        acc[4] += Ai * eab * lj1v * (-r / (sab * sab)) * 0.5;
+        if (eab > 0) acc[5] += Ai * ljcap0(rs) * sv[1] / (2 * eab);
+      }
+    }
+    if (mode != 0) {
+      const double b = B[e];
+      fr += b * EPSILON0INV * s[5] * G * G * cl1(G * r);
+      acc[3] += b * EPSILON0INV * G * cl0(G * r);
+    }
+    acc[0] += fr * gx;
+    acc[1] += fr * gy;
+    acc[2] += fr * gz;
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int p = 0; p < 6; ++p) {
+    const double v = warp_sum(acc[p]);
+    if (lane == 0) red[p][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double v = red[threadIdx.x][0];
+    for (int i = 1; i < NT / 64; ++i) v += red[threadIdx.x][i];
+    part[((int64_t)a * gridDim.x + blockIdx.x) * 6 + threadIdx.x] = v;
+  }
+}
+
+// tau term of the adjoint gradient: with S(k) = w(kx) s(k) sum_i tau_i(k)
+// conj(Lambda_i(k)) (Lambda = r2c(lambda), w = 1 at kx = 0 and N0/2, else
+// 2), -tau_scale Re sum_k S(k) dF(k)/dtheta_a for F = sum_a Q_a e^{-ik.x_a}:
+// theta = Q_a -> e^{-ik.x_a}, x_a -> -i k Q_a e^{-ik.x_a}.  Site a =
+// blockIdx.y, chunk of K = blockIdx.x.  part[(a * gridDim.x + chunk) * 4 + p]
+// for p = x, y, z, q.
+__global__ void __launch_bounds__(NT)
+grad_tau(GridArgs g, SitesDesc sd, const cpx<double>* __restrict__ Lam,
+         const double* __restrict__ tau3, double* __restrict__ part) {
+  __shared__ double red[4][NT / 64];
+  const int a = blockIdx.y;
+  const double* s = sd.xyz_s_e_q + 6 * a;
+  double acc[4] = {0, 0, 0, 0};
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.K; e += stride) {
+    const int x = (int)(e % g.NH);
+    const int y = (int)((e / g.NH) % g.N1);
+    const int z = (int)(e / ((int64_t)g.NH * g.N1));
+    const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
+    const int iz = (z <= g.N2 / 2) ? z : z - g.N2;
+    const double kx = x * g.dkx, ky = iy * g.dky, kz = iz * g.dkz;
+    double sre = 0.0, sim = 0.0;
+    for (int i = 0; i < g.m; ++i) {
+      const cpx<double> L = Lam[i * g.K + e];
+      const double ti = tau3[i * g.K + e];
+      sre += ti * L.re;
+      sim -= ti * L.im;
+    }
+    const double w = ((x == 0 || x == g.N0 / 2) ? 1.0 : 2.0) * (((x + iy + iz) & 1) ? -1.0 : 1.0);
+    sre *= w;
+    sim *= w;
+    double sn, cs;
+    sincos(kx * s[0] + ky * s[1] + kz * s[2], &sn, &cs);
+    // S e^{-i phi} = (sre + i sim)(cs - i sn)
+    const double pre = sre * cs + sim * sn;
+    const double pim = sim * cs - sre * sn;
+    acc[3] += pre;                       // Re S e^{-i phi}
+    // Re[S e^{-i phi} (-i k Q)] = Q k Im[S e^{-i phi}]
+    acc[0] += s[5] * kx * pim;
+    acc[1] += s[5] * ky * pim;
+    acc[2] += s[5] * kz * pim;
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const double v = warp_sum(acc[p]);
+    if (lane == 0) red[p][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double v = red[threadIdx.x][0];
+    for (int i = 1; i < NT / 64; ++i) v += red[threadIdx.x][i];
+    part[((int64_t)a * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = -g.tau_scale * v;
   }
 }
 

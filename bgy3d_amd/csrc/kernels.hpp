@@ -31,7 +31,9 @@ struct SoluteDesc {
   const void* Fs;            // [N2][N1][NH] form factor times s(k), cpx<T>
   double* part;              // per residual workgroup: max|r|, sum r^2, dots
   int active;                // 0: converged / failed / waiting: return
-  int pad_;
+  int mode;                  // 0 F(t); 1 adjoint D K l + g - l; 2 K l; 3 spectrum
+  const double* dt;          // mode 1: D = dc/dt at the converged t (FP64)
+  const double* gv;          // mode 1: g = dmu/dt
 };
 constexpr int NPART = 2 + MAX_DIIS;   // doubles per residual workgroup
 

@@ -218,6 +218,8 @@ struct SoluteState {
   DevBuf<cpx<float>> fs32;
   DevBuf<double> part;     // residual-workgroup partials, NPART each
   DevBuf<double> sites;    // n * 6
+  DevBuf<double> Dt, gmu;  // adjoint: dc/dt and dmu/dt at the converged t
+  double tol_lin = 0.0;    // adjoint: absolute tolerance (relative to |g|)
   int n_sites = 0;
   // host MDIIS state
   std::vector<int> order;     // history slots, oldest first
@@ -335,6 +337,8 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
 struct Config {
   int solver, precision, polish, depth, verbose, max_iter;
   double lam, tol, mixed_tol;
+  int mode = 0;          // 0: F(t) = 0; 1: adjoint lambda = D K lambda + g
+  double rel_tol = 0.0;  // mode 1: stop at max|r| <= rel_tol max|g|
 };
 
 // Copy the iterate between its FP32 and FP64 buffers for the solutes whose
@@ -419,6 +423,9 @@ void iterate_batch(Problem& pb, const Config& cf) {
       d.W = s.W.p;
       d.Fs = fp32 ? (const void*)s.fs32.p : (const void*)s.fs64.p;
       d.part = s.part.p;
+      d.mode = cf.mode;
+      d.dt = s.Dt.p;
+      d.gv = s.gmu.p;
     }
     evaluate(pb, fp32, sd);
     HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, pb.s));
@@ -460,9 +467,11 @@ void iterate_batch(Problem& pb, const Config& cf) {
           continue;  // the same iterate is evaluated again in FP64
         }
       } else {
+        if (cf.mode == 1 && s.iters == 1) s.tol_lin = cf.rel_tol * rinf;  // r(0) = g
+        const double tol = cf.mode == 1 ? s.tol_lin : cf.tol;
         const bool polish_done = cf.precision == RISM3D_MIXED && polish_left <= 0;
-        if (rinf <= cf.tol || polish_done) {
-          s.status = (rinf <= cf.tol || (cf.precision == RISM3D_MIXED && s.reached_mixed))
+        if (rinf <= tol || polish_done) {
+          s.status = (rinf <= tol || (cf.precision == RISM3D_MIXED && s.reached_mixed))
                          ? RISM3D_OK : RISM3D_NONCONVERGED;
           s.active = 0;
           continue;
@@ -549,6 +558,157 @@ int validate(const Inputs& in) {
     if (lds64 > 64 * 1024) return RISM3D_UNSUPPORTED;
   }
   return RISM3D_OK;
+}
+
+
+// Gradients after a converged solve (t in t64 of every solute).
+// TPT ('free-energy-gradient, hnc3d.c:1666-1742) for res[b].gradient, and
+// the exact adjoint d mu / d(x, y, z, q, sigma, epsilon) for
+// res[b].dmu_dsite: lambda = D_t K lambda + dmu/dt by linear MDIIS, then
+// K lambda, Lambda = r2c(lambda), the coefficient fields and two
+// reductions (grad_field, grad_tau).
+void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& cf0,
+               const std::vector<SitesDesc>& sites) {
+  const GridArgs& g = pb.g;
+  const rism3d_problem& pd = *in.pd;
+  const int B = pb.B;
+  const int64_t mP = pb.mP;
+  std::vector<double> sse(3 * g.m);
+  for (int i = 0; i < g.m; ++i) {
+    sse[3 * i] = in.solvent[i].sigma;
+    sse[3 * i + 1] = in.solvent[i].epsilon;
+    sse[3 * i + 2] = in.solvent[i].charge;
+  }
+  bool want_tpt = false, want_adj = false;
+  for (int b = 0; b < B; ++b) {
+    want_tpt |= res[b].gradient != nullptr && pb.st[b].status == RISM3D_OK;
+    want_adj |= res[b].dmu_dsite != nullptr && pb.st[b].status == RISM3D_OK;
+  }
+  if (!want_tpt && !want_adj) return;
+  DevBuf<double> d_sse(sse.size());
+  HIPCHK(hipMemcpyAsync(d_sse.p, sse.data(), sse.size() * 8, hipMemcpyHostToDevice, pb.s));
+  const int cp = (int)std::min<int64_t>(512, (g.P + NT - 1) / NT);
+  const int ck = (int)std::min<int64_t>(512, (g.K + NT - 1) / NT);
+  DevBuf<double> A(mP), Bf(g.P);
+
+  auto field_reduce = [&](int b, int mode, std::vector<double>& acc6) {
+    const int n = sites[b].n;
+    DevBuf<double> part((size_t)n * cp * 6);
+    hipLaunchKernelGGL(grad_field, dim3(cp, n), dim3(NT), 0, pb.s, g, sites[b], d_sse.p, A.p, Bf.p,
+                       pd.L[0], pd.L[1], pd.L[2], pb.h[0], pb.h[1], pb.h[2], mode, part.p);
+    std::vector<double> hp(part.n);
+    HIPCHK(hipMemcpyAsync(hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+    acc6.assign((size_t)n * 6, 0.0);
+    for (int a = 0; a < n; ++a)
+      for (int c = 0; c < cp; ++c)
+        for (int p = 0; p < 6; ++p) acc6[a * 6 + p] += hp[((size_t)a * cp + c) * 6 + p];
+  };
+
+  if (want_tpt) {
+    for (int b = 0; b < B; ++b) {
+      if (!res[b].gradient || pb.st[b].status != RISM3D_OK) continue;
+      hipLaunchKernelGGL(adj_coef, dim3(cp), dim3(NT), 0, pb.s, g, sites[b], pb.st[b].t64.p,
+                         pb.st[b].bv64.p, (const double*)nullptr, pd.L[0], pd.L[1], pd.L[2],
+                         pb.h[0], pb.h[1], pb.h[2], pd.rho, 0, A.p, Bf.p, (double*)nullptr,
+                         (double*)nullptr);
+      std::vector<double> acc;
+      field_reduce(b, 0, acc);
+      for (int a = 0; a < sites[b].n; ++a)
+        for (int d = 0; d < 3; ++d) res[b].gradient[a * 3 + d] = acc[a * 6 + d];
+    }
+  }
+  if (!want_adj) return;
+  if (!g.renorm) {
+    for (int b = 0; b < B; ++b)
+      if (res[b].dmu_dsite)
+        for (int q = 0; q < sites[b].n * 6; ++q) res[b].dmu_dsite[q] = NAN;
+    return;
+  }
+  // adjoint solve for every solute that wants it, batched
+  std::vector<DevBuf<double>> tsave(B);
+  for (int b = 0; b < B; ++b) {
+    auto& s = pb.st[b];
+    const bool want = res[b].dmu_dsite && s.status == RISM3D_OK;
+    s.active = want;
+    if (!want) continue;
+    s.Dt = DevBuf<double>(mP);
+    s.gmu = DevBuf<double>(mP);
+    tsave[b] = DevBuf<double>(mP);
+    HIPCHK(hipMemcpyAsync(tsave[b].p, s.t64.p, mP * 8, hipMemcpyDeviceToDevice, pb.s));
+    hipLaunchKernelGGL(adj_coef, dim3(cp), dim3(NT), 0, pb.s, g, sites[b], s.t64.p, s.bv64.p,
+                       (const double*)nullptr, pd.L[0], pd.L[1], pd.L[2], pb.h[0], pb.h[1],
+                       pb.h[2], pd.rho, 1, (double*)nullptr, (double*)nullptr, s.Dt.p, s.gmu.p);
+    HIPCHK(hipMemsetAsync(s.t64.p, 0, mP * 8, pb.s));
+    s.iters = s.iters32 = 0;
+    s.best = INFINITY;
+    s.since_best = 0;
+    s.status = RISM3D_NONCONVERGED;
+  }
+  Config cf = cf0;
+  cf.mode = 1;
+  cf.precision = RISM3D_FP64;
+  cf.solver = RISM3D_SOLVER_MDIIS;
+  cf.lam = 0.5;
+  cf.rel_tol = 1e-11;
+  iterate_batch(pb, cf);
+  for (int b = 0; b < B; ++b) {
+    auto& s = pb.st[b];
+    if (!tsave[b].p) continue;
+    res[b].adjoint_iterations = s.iters;
+    const int n = sites[b].n;
+    if (s.status != RISM3D_OK) {
+      for (int q = 0; q < n * 6; ++q) res[b].dmu_dsite[q] = NAN;
+      continue;
+    }
+    // K lambda (mode 2) into R, then Lambda = r2c(lambda) (mode 3) into W
+    std::vector<SoluteDesc> sd(B);
+    for (auto& d : sd) std::memset(&d, 0, sizeof d);
+    SoluteDesc& d = sd[b];
+    d.active = 1;
+    d.t = s.t64.p;
+    d.r = s.R.p;
+    d.unew = s.U.p;
+    d.bv = s.bv64.p;
+    d.W = s.W.p;
+    d.Fs = s.fs64.p;
+    d.part = s.part.p;
+    d.mode = 2;
+    evaluate(pb, false, sd);
+    d.mode = 3;
+    {
+      // forward passes only: closure_r2c (identity), fft_y, zstar (forward)
+      HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), B * sizeof(SoluteDesc), hipMemcpyHostToDevice, pb.s));
+      using T = double;
+      constexpr int LN = y_lines<T>();
+      const dim3 gx(resid_grid_x(g), g.m, B);
+      const dim3 gy((g.NH + LN - 1) / LN, g.m * g.N2, B);
+      launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
+      launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
+      launch_zstar<T>(g.N2, g.NH, g.N1, B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p);
+    }
+    hipLaunchKernelGGL(adj_coef, dim3(cp), dim3(NT), 0, pb.s, g, sites[b], tsave[b].p, s.bv64.p,
+                       (const double*)s.R.p, pd.L[0], pd.L[1], pd.L[2], pb.h[0], pb.h[1],
+                       pb.h[2], pd.rho, 2, A.p, Bf.p, (double*)nullptr, (double*)nullptr);
+    std::vector<double> acc;
+    field_reduce(b, 1, acc);
+    DevBuf<double> tpart((size_t)n * ck * 4);
+    hipLaunchKernelGGL(grad_tau, dim3(ck, n), dim3(NT), 0, pb.s, g, sites[b],
+                       (const cpx<double>*)s.W.p, pb.tau3.p, tpart.p);
+    std::vector<double> hp(tpart.n);
+    HIPCHK(hipMemcpyAsync(hp.data(), tpart.p, hp.size() * 8, hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipMemcpyAsync(s.t64.p, tsave[b].p, mP * 8, hipMemcpyDeviceToDevice, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+    HIPCHK(hipGetLastError());
+    for (int a = 0; a < n; ++a) {
+      double tp[4] = {0, 0, 0, 0};
+      for (int c = 0; c < ck; ++c)
+        for (int p = 0; p < 4; ++p) tp[p] += hp[((size_t)a * ck + c) * 4 + p];
+      for (int p = 0; p < 6; ++p) res[b].dmu_dsite[a * 6 + p] = acc[a * 6 + p];
+      for (int p = 0; p < 4; ++p) res[b].dmu_dsite[a * 6 + p] += tp[p];
+    }
+    s.status = RISM3D_OK;
+  }
 }
 
 std::mutex g_mutex;  // one solve per process at a time (shared streams)
@@ -817,6 +977,8 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
       }
     }
   }
+  gradients(pb, in, res, cf, sites);
+
   int worst = RISM3D_OK;
   for (int b = 0; b < B; ++b) worst = std::max(worst, res[b].status);
   return worst;

@@ -73,7 +73,10 @@ class Result(ctypes.Structure):
                 ("iterations", ctypes.c_int), ("iterations_fp32", ctypes.c_int),
                 ("status", ctypes.c_int),
                 ("g", ctypes.POINTER(ctypes.c_double)),
-                ("t", ctypes.POINTER(ctypes.c_double))]
+                ("t", ctypes.POINTER(ctypes.c_double)),
+                ("gradient", ctypes.POINTER(ctypes.c_double)),
+                ("dmu_dsite", ctypes.POINTER(ctypes.c_double)),
+                ("adjoint_iterations", ctypes.c_int)]
 
 
 EXPORTS = ("rism3d_options_default", "rism3d_solute_solve", "rism3d_batch_solve",
@@ -201,14 +204,18 @@ def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=False):
+def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=False,
+                     derivatives=False, adjoint=False):
     """Solve the 3D-RISM solute problem on the GPU.
 
     Returns a dict like the reference alist: 'free-energy',
     'partial-molar-volume-factor', 'GUV' (g, [m][N2][N1][N0]), plus
     'free-energy-hnc'/'-kh'/'-gf', 'iterations', 'residual', 'RESTART'
     (opaque token, pass back as ``restart=`` and free with
-    ``restart_destroy``) if ``restart`` is not None."""
+    ``restart_destroy``) if ``restart`` is not None.
+    derivatives=True adds 'free-energy-gradient' (n, 3), the reference's
+    TPT gradient (--derivatives); adjoint=True adds 'free-energy-dsite'
+    (n, 6), exact d mu / d(x, y, z, q, sigma, epsilon) by the adjoint."""
     lib = load()
     pd = problem_data(settings)
     opt = make_options(settings)
@@ -223,6 +230,13 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
     if want_t:
         t = np.empty(shape)
         res.t = _dptr(t)
+    grad = dsite = None
+    if derivatives:
+        grad = np.full((len(su), 3), np.nan)
+        res.gradient = _dptr(grad)
+    if adjoint:
+        dsite = np.full((len(su), 6), np.nan)
+        res.dmu_dsite = _dptr(dsite)
     chi_p = None
     if chi is not None:
         chi = np.ascontiguousarray(chi, dtype=np.float64)
@@ -246,6 +260,11 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
     }
     if t is not None:
         out["t"] = t
+    if grad is not None:
+        out["free-energy-gradient"] = grad
+    if dsite is not None:
+        out["free-energy-dsite"] = dsite
+        out["adjoint-iterations"] = res.adjoint_iterations
     if tok is not None:
         out["RESTART"] = tok.value
     return out

@@ -120,6 +120,15 @@ typedef struct rism3d_result {
                           g_i = h_i + 1 on the reference grid, r = i h - L/2 */
   double *t;           /* in: NULL or caller buffer [m][N2][N1][N0]; out:
                           converged indirect correlation t_i                */
+  double *gradient;    /* in: NULL or [n][3]; out: the reference's TPT
+                          gradient rho h^3 sum_i <g_i | dv_i/dx_a>
+                          ('free-energy-gradient, --derivatives;
+                          gradients()/energy1() hnc3d.c:1666-1742)          */
+  double *dmu_dsite;   /* in: NULL or [n][6]; out: exact d mu / d(x, y, z,
+                          q, sigma, epsilon) of every solute site by the
+                          adjoint of the residual (renorm path only; no
+                          counterpart in the reference)                     */
+  int adjoint_iterations;  /* linear MDIIS steps of the adjoint solve      */
 } rism3d_result;
 
 /* Opaque restart token: the converged t (hnc3d.c:1826-1845, 2299-2316). */

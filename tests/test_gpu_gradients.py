@@ -1,0 +1,81 @@
+"""GPU gradients against the oracle.
+
+* TPT gradient (the reference's --derivatives, hnc3d.c:1666-1742) vs
+  oracle/gradients.py tpt_gradient: 1e-9 relative.
+* Exact adjoint d mu / d(x, y, z, q, sigma, epsilon) vs the oracle's
+  adjoint (itself checked against finite differences of mu on the CPU,
+  tests/test_oracle_gradients.py) to 1e-6 relative, and one direct central
+  finite difference of the GPU mu (h = 1e-4) to 1e-4 relative, the bar of
+  the north star.
+"""
+import numpy as np
+import pytest
+
+from conftest import README
+from oracle import closures as C
+from oracle import gradients as Gr
+from oracle import rism3d as O
+
+pytestmark = pytest.mark.gpu
+W = "water, PR-SPC/E"
+
+
+def _solute():
+    from bgy3d_amd import synthetic as S
+    return list(S.solute(5, n=4))
+
+
+def _settings(closure):
+    return dict(README, N=32, L=8.0, rmax=40.0, nrad=1536, closure=closure,
+                **{"norm-tol": 1e-12, "max-iter": 2000})
+
+
+@pytest.mark.parametrize("closure", ["HNC", "KH"])
+def test_gradients_vs_oracle(closure, chi_w):
+    print(f"case gradients {closure}", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd.solutes import find_molecule
+    sol = _solute()
+    s = _settings(closure)
+    out = R.hnc3d_run_solute(sol, W, s, chi=chi_w, derivatives=True, adjoint=True)
+    pd = O.Problem.from_cli(8.0, 32, README["beta"], README["rho"], closure=C.NAMES[closure],
+                            rmax=40.0, nrad=1536)
+    ref = O.Solute3D(pd, find_molecule(W), sol, chi_w)
+    t, _ = ref.solve(atol=1e-12)
+    obs = ref.observables(t)
+    tpt = Gr.tpt_gradient(pd, find_molecule(W), sol, obs["g"])
+    adj, _ = Gr.adjoint_gradient(ref, t)
+    g_gpu = out["free-energy-gradient"]
+    d_gpu = out["free-energy-dsite"]
+    e_tpt = np.max(np.abs(g_gpu - tpt)) / np.max(np.abs(tpt))
+    e_adj = np.max(np.abs(d_gpu - adj) / np.maximum(np.abs(adj), 1e-3))
+    print(f"  TPT rel {e_tpt:.2e}  adjoint rel {e_adj:.2e}  adj iters {out['adjoint-iterations']}",
+          flush=True)
+    assert e_tpt < 1e-9
+    assert e_adj < 1e-6
+
+
+def test_adjoint_vs_finite_difference(chi_w):
+    print("case adjoint-fd", flush=True)
+    from bgy3d_amd import rism3d as R
+    sol = _solute()
+    s = _settings("HNC")
+    out = R.hnc3d_run_solute(sol, W, s, chi=chi_w, adjoint=True)
+    h = 1e-4
+    for a, p in ((0, 0), (1, 3), (2, 4), (3, 5)):
+        def moved(sign):
+            x = [[n, list(xyz), sg, ep, q] for n, xyz, sg, ep, q in sol]
+            if p < 3:
+                x[a][1][p] += sign * h
+            elif p == 3:
+                x[a][4] += sign * h
+            elif p == 4:
+                x[a][2] += sign * h
+            else:
+                x[a][3] += sign * h
+            return [(n, tuple(xyz), sg, ep, q) for n, xyz, sg, ep, q in x]
+        fd = (R.hnc3d_run_solute(moved(+1), W, s, chi=chi_w)["free-energy"]
+              - R.hnc3d_run_solute(moved(-1), W, s, chi=chi_w)["free-energy"]) / (2 * h)
+        adj = out["free-energy-dsite"][a, p]
+        print(f"  site {a} param {p}: adjoint {adj:.8f} fd {fd:.8f}", flush=True)
+        assert abs(adj - fd) <= 1e-4 * max(abs(fd), 1e-2)

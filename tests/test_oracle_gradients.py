@@ -1,0 +1,51 @@
+"""Pin oracle/gradients.py: the TPT field derivative (field1's ljcap1 and
+ccap1, bgy3d-solutes.c:381-468) against central differences of the full
+capped site potential, and the adjoint d mu / d theta against central
+differences of converged mu, all six parameter kinds."""
+import numpy as np
+
+from conftest import CHI_W, load_chi
+from oracle import gradients as Gr
+from oracle import rism3d as O
+from bgy3d_amd.solutes import find_molecule
+from bgy3d_amd import synthetic as S
+
+W = find_molecule("water, PR-SPC/E")
+
+
+def test_capped_derivatives_fd():
+    r = np.linspace(0.01, 3.0, 400)
+    r = r[np.abs(r - 0.2) > 1e-3]
+    h = 1e-6
+    for f0, f1 in ((O.ljcap0, Gr.ljcap1), (O.cscap0, Gr.cscap1), (O.cl0, Gr.cl1), (Gr.ccap0, Gr.ccap1)):
+        fd = (f0(r + h) - f0(r - h)) / (2 * h)
+        np.testing.assert_allclose(f1(r), fd, rtol=1e-5, atol=1e-4)
+
+
+def test_adjoint_matches_fd():
+    chi = load_chi(CHI_W)
+    sol = list(S.solute(5, n=4))
+    pd = O.Problem.from_cli(8.0, 32, 1.6889, 0.0333295, rmax=40.0, nrad=1536)
+
+    def mu(sites):
+        s = O.Solute3D(pd, W, sites, chi)
+        t, _ = s.solve(atol=1e-12)
+        return s.observables(t)["mu"], s, t
+
+    _, s0, t0 = mu(sol)
+    grad, _ = Gr.adjoint_gradient(s0, t0)
+    h = 1e-4
+    for a, p in ((0, 1), (1, 3), (2, 4), (3, 5)):
+        def moved(sign):
+            x = [[n, list(xyz), sg, ep, q] for n, xyz, sg, ep, q in sol]
+            if p < 3:
+                x[a][1][p] += sign * h
+            elif p == 3:
+                x[a][4] += sign * h
+            elif p == 4:
+                x[a][2] += sign * h
+            else:
+                x[a][3] += sign * h
+            return [(n, tuple(xyz), sg, ep, q) for n, xyz, sg, ep, q in x]
+        fd = (mu(moved(+1))[0] - mu(moved(-1))[0]) / (2 * h)
+        assert abs(grad[a, p] - fd) <= 1e-5 * max(abs(fd), 1e-2), (a, p, grad[a, p], fd)
