@@ -83,6 +83,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nsolutes", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 512 MiB device-to-device copy (3 launches), the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -92,6 +95,16 @@ def main():
 
     from bgy3d_amd import rism3d as R
     from bgy3d_amd import synthetic as S
+
+    if args.calibrate_copy:
+        import torch
+        n = 64 * 1024 * 1024  # doubles: 512 MiB
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     solutes = S.batch(range(args.nsolutes))
     L_full = S.forced_box(solutes, N=128)
