@@ -238,3 +238,17 @@ def test_cli_energy_hnc(capsys):
     print(f"  printed {out}", flush=True)
     assert rc == 0
     assert abs(float(out) - (-6.349052)) < 1e-6
+
+
+def test_forces_scm_anchors_gpu():
+    """test/forces.scm:28-30 through the C ABI with chi_k == NULL: the C++
+    1D producer makes chi on the upscaled grid, then the 3D solve."""
+    print("case forces-scm-96", flush=True)
+    R = _R()
+    s = {"N": 96, "L": 10.0, "rho": 0.0333295, "beta": 1.6889, "closure": "HNC",
+         "norm-tol": 1e-12, "max-iter": 2000}
+    for name, ref in (("OW", 6.86940765594207), ("OW2-UA", 5.82278007673305),
+                      ("OW2", 14.0074968008389)):
+        out = R.hnc3d_run_solute(name, "OW", s, chi=None)
+        print(f"  {name}: mu {out['free-energy']:.12f} reference {ref}", flush=True)
+        assert abs(out["free-energy"] - ref) < 1e-8

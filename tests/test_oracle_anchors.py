@@ -72,3 +72,17 @@ def test_axis_permutation_symmetry():
     assert abs(a["mu"] - b["mu"]) < 1e-9
     # g permutes with the coordinates: g'(z, y, x) = g(y, x, z)
     np.testing.assert_allclose(b["g"], np.transpose(a["g"], (0, 2, 3, 1)), atol=1e-8)
+
+
+def test_forces_scm_anchors():
+    """test/forces.scm:28-30: HNC, --L 10 --N 96, OW solvent, chi from the
+    1D run hnc3d_solute_solve makes itself (upscaled rmax 40, nrad 1536,
+    plain RISM): mu(OW2, d/2 = 2) = 14.0074968008389, mu(OW) =
+    6.86940765594207, mu(OW2-UA) = 5.82278007673305."""
+    ow = find_molecule("OW")
+    vv = rism1d.rism_vv(C.HNC, 1536, 40.0, 1.6889, 0.0333295, ow)
+    pd = rism3d.Problem.from_cli(10.0, 96, 1.6889, 0.0333295, rmax=40.0, nrad=1536)
+    for name, ref in (("OW", 6.86940765594207), ("OW2-UA", 5.82278007673305),
+                      ("OW2", 14.0074968008389)):
+        out = rism3d.solute_solve(pd, ow, find_molecule(name), vv["chi"], atol=1e-12)
+        assert abs(out["mu"] - ref) < 1e-8, (name, out["mu"])
