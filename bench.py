@@ -31,12 +31,12 @@ HBM_PEAK_GBS = 8000.0
 W = "water, PR-SPC/E"
 
 
-def settings_for(L_full, precision):
+def settings_for(L_full, precision, depth=8, step=0.5, polish=3):
     s = {"N": 128, "L": L_full / 2, "rho": 0.0333295, "beta": 1.6889,
          "rmax": 160.0, "nrad": 4096, "closure": "KH", "max-iter": 3000,
-         "norm-tol": 1e-10, "diis-depth": 8, "diis-step": 0.5}
+         "norm-tol": 1e-10, "diis-depth": depth, "diis-step": step}
     if precision == "mixed":
-        s.update({"precision": "mixed", "mixed-tol": 1e-5, "polish-iters": 3})
+        s.update({"precision": "mixed", "mixed-tol": 1e-5, "polish-iters": polish})
     return s
 
 
@@ -83,6 +83,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nsolutes", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--diis-depth", type=int, default=8)
+    ap.add_argument("--diis-step", type=float, default=0.5)
+    ap.add_argument("--polish", type=int, default=3)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 512 MiB device-to-device copy (3 launches), the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -109,8 +112,8 @@ def main():
     solutes = S.batch(range(args.nsolutes))
     L_full = S.forced_box(solutes, N=128)
     chi = chi_solvent()
-    mixed = settings_for(L_full, "mixed")
-    fp64 = settings_for(L_full, "fp64")
+    mixed = settings_for(L_full, "mixed", args.diis_depth, args.diis_step, args.polish)
+    fp64 = settings_for(L_full, "fp64", args.diis_depth, args.diis_step)
 
     for _ in range(args.warmup):
         R.batch_solve(solutes, W, mixed, chi)
@@ -132,6 +135,7 @@ def main():
     value = n / t_mixed
     fp64_rate = n / t_fp64
 
+    worst = int(np.argmax(dmu))
     dom = max(prof, key=lambda k: prof[k][0])
     ms, by, nl = prof[dom]
     achieved = by / (ms * 1e-3) / 1e9
@@ -153,7 +157,7 @@ def main():
                                f"in PR-SPC/E water, KH + tau renorm, 128^3 grid, box {L_full} A, "
                                "one batched solve per step",
                    "grid": 128, "box_A": L_full, "solutes": n, "closure": "KH",
-                   "mixed_tol": 1e-5, "polish_iters": 3, "fp64_tol": 1e-10},
+                   "mixed_tol": 1e-5, "polish_iters": args.polish, "fp64_tol": 1e-10},
         "fp64_solves_per_s": fp64_rate,
         "mixed_over_fp64": value / fp64_rate,
         "max_abs_dmu_kcal": max(dmu),
@@ -161,6 +165,13 @@ def main():
         "mean_iters_mixed": float(np.mean([a["iterations"] for a in res_mixed])),
         "mean_iters_fp32": float(np.mean([a["iterations-fp32"] for a in res_mixed])),
         "mean_iters_fp64": fp64_iters,
+        "worst_solute": {"seed": worst, "dmu": dmu[worst],
+                         "mixed": {k: res_mixed[worst][k] for k in ("free-energy", "iterations",
+                                                                    "iterations-fp32", "residual",
+                                                                    "status")},
+                         "fp64": {k: res_fp64[worst][k] for k in ("free-energy", "iterations",
+                                                                  "residual", "status")}},
+        "diis": {"depth": args.diis_depth, "step": args.diis_step, "polish": args.polish},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                      "avg_launch_us": ms / nl * 1e3, "bytes_per_launch": by / nl},

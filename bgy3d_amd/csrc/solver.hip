@@ -234,6 +234,7 @@ struct SoluteState {
   int reached_mixed = 0;      // FP32 phase ended at mixed_tol or its floor
   double best32 = INFINITY;
   int stall32 = 0;
+  int polish = 0;             // FP64 steps taken after the FP32 phase
   double resid = INFINITY, best = INFINITY;
   int since_best = 0;
 };
@@ -378,7 +379,6 @@ void iterate_batch(Problem& pb, const Config& cf) {
     s.coef.clear();
   }
   if (cf.precision == RISM3D_MIXED) convert_iterates(pb, std::vector<int>(B, 1), false);
-  int polish_left = cf.polish;
   for (;;) {
     bool any = false, any32 = false;
     for (int b = 0; b < B; ++b) {
@@ -454,9 +454,14 @@ void iterate_batch(Problem& pb, const Config& cf) {
         // FP32 hands over at mixed_tol, or once its residual stops falling
         // (the FP32 floor, ~1e-7 max|t|, can lie above mixed_tol near
         // charged sites): no 30% gain on the best value in 12 iterations.
-        if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; } else { s.stall32++; }
+        // Stalls count only near the floor (max|F| <= 1e-3): higher up a
+        // slow phase is not a floor, and handing over there would leave
+        // the FP64 polish far from the root.
+        if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; }
+        else if (rinf <= 1e-3) { s.stall32++; }
         if (rinf <= cf.mixed_tol || s.stall32 >= 12 || s.iters32 >= cf.max_iter) {
           s.reached_mixed = rinf <= cf.mixed_tol || s.stall32 >= 12;
+          s.polish = 0;
           s.phase32 = 0;
           s.order.clear();
           s.coef.clear();
@@ -468,14 +473,20 @@ void iterate_batch(Problem& pb, const Config& cf) {
         }
       } else {
         if (cf.mode == 1 && s.iters == 1) s.tol_lin = cf.rel_tol * rinf;  // r(0) = g
+        // FP64 and the MIXED polish both aim at norm_tol; the polish stops
+        // after polish_iters steps and is OK if it reached mixed_tol.
         const double tol = cf.mode == 1 ? s.tol_lin : cf.tol;
-        const bool polish_done = cf.precision == RISM3D_MIXED && polish_left <= 0;
-        if (rinf <= tol || polish_done) {
-          s.status = (rinf <= tol || (cf.precision == RISM3D_MIXED && s.reached_mixed))
-                         ? RISM3D_OK : RISM3D_NONCONVERGED;
+        if (rinf <= tol) {
+          s.status = RISM3D_OK;
           s.active = 0;
           continue;
         }
+        if (cf.precision == RISM3D_MIXED && s.polish >= cf.polish) {
+          s.status = rinf <= cf.mixed_tol ? RISM3D_OK : RISM3D_NONCONVERGED;
+          s.active = 0;
+          continue;
+        }
+        s.polish++;
         if (s.iters - s.iters32 >= cf.max_iter) {
           s.status = RISM3D_NONCONVERGED;
           s.active = 0;
@@ -507,7 +518,6 @@ void iterate_batch(Problem& pb, const Config& cf) {
       }
     }
     if (any_handover) convert_iterates(pb, handover, true);
-    if (!fp32 && cf.precision == RISM3D_MIXED) polish_left--;
   }
   // solutes that stopped inside the FP32 phase (NaN, limits): FP64 copy
   std::vector<int> mask(B, 0);

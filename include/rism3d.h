@@ -96,8 +96,12 @@ typedef struct rism3d_options {
   int renorm;          /* 1: tau renormalisation (default), 0: --no-renorm */
   int solver;          /* enum rism3d_solver                               */
   int precision;       /* enum rism3d_precision                            */
-  int polish_iters;    /* MIXED: max FP64 iterations after the FP32 phase  */
-  double mixed_tol;    /* MIXED: max-norm residual at which FP32 hands over */
+  int polish_iters;    /* MIXED: FP64 steps after the FP32 phase, towards
+                          norm_tol; NONCONVERGED if the last one is above
+                          mixed_tol                                        */
+  double mixed_tol;    /* MIXED: max-norm residual at which FP32 hands over
+                          (or its FP32 floor, detected as a stall below
+                          1e-3)                                            */
   int diis_depth;      /* MDIIS history length (1..16)                     */
   double diis_step;    /* MDIIS step lambda applied to residuals           */
   int verbose;         /* print per-iteration norms to stderr              */
