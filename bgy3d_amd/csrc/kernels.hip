@@ -216,8 +216,10 @@ ktab3(GridArgs g, const double* __restrict__ chi_tab, const double* __restrict__
 }
 
 // --------------------------- 3. z c2c + chi contraction + tau + z c2c^-1
-// Grid: x = solute (fastest: concurrent workgroups share one chi3/tau3
-// tile), y = ceil(NH / LINES), z = ky row.  Dynamic LDS:
+// Grid: zorder 0: x = solute (concurrent workgroups share one chi3/tau3
+// tile), y = ceil(NH / LINES), z = ky row; zorder 1: x = kx tile, y = ky
+// row, z = solute (concurrent workgroups stream contiguous planes of one
+// solute).  Dynamic LDS:
 // m * LINES * (N2 + 1) complex.  chi3: [m(m+1)/2][K] chi_ij - delta_ij;
 // tau3: [m][K] s_i(|k|), both on the k grid, in the working type.
 template <typename T, int N2, int LINES>
@@ -227,11 +229,13 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   constexpr int LS = N2 + 1;
   extern __shared__ unsigned char smem[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-  const SoluteDesc& d = sd[blockIdx.x];
+  // zorder 0: grid (solute, kx tile, ky); 1: grid (kx tile, ky, solute)
+  const int b = g.zorder ? blockIdx.z : blockIdx.x;
+  const SoluteDesc& d = sd[b];
   if (!d.active) return;
   const int m = g.m;
-  const int y = blockIdx.z;
-  const int kx0 = blockIdx.y * LINES;
+  const int y = g.zorder ? blockIdx.y : blockIdx.z;
+  const int kx0 = (g.zorder ? blockIdx.x : blockIdx.y) * LINES;
   const int64_t zs = (int64_t)g.N1 * g.NH;  // z stride
   const int64_t base = (int64_t)y * g.NH + kx0;
   cpx<T>* W = static_cast<cpx<T>*>(d.W) + base;

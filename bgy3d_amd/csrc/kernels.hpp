@@ -53,6 +53,7 @@ struct GridArgs {
   int renorm;          // 1: tau, 0: Ng long-range form
   int closure;         // enum rism3d_closure
   int pse_order;       // n of PSE-n, 1 for KH
+  int zorder;          // zstar grid order (see zstar)
   double q[MAX_SITES]; // solvent charges
 };
 

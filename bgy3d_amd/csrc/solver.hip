@@ -87,9 +87,17 @@ void launch_zstar_l(int N2, dim3 grid, size_t lds, hipStream_t s, const SoluteDe
 #undef X
   }
 }
+// Tuning knobs, read once: RISM3D_ZSTAR_LINES (kx columns per zstar
+// workgroup, 2..16) and RISM3D_ZSTAR_ORDER (0 or 1, see zstar).
+int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atoi(v) : dflt;
+}
 template <typename T>
 int zstar_lines(int N2, int m) {
   int L = (sizeof(T) == 8) ? 8 : 16;
+  const int want = env_int("RISM3D_ZSTAR_LINES", 0);
+  if (want == 2 || want == 4 || want == 8 || want == 16) L = want;
   while (L > 2 && (size_t)m * L * (N2 + 1) * sizeof(cpx<T>) > 64 * 1024) L /= 2;
   return L;
 }
@@ -98,7 +106,7 @@ void launch_zstar(int N2, int NH, int N1, int B, hipStream_t s, const SoluteDesc
                   const GridArgs& g, const cpx<T>* tw, const T* chi, const T* tau) {
   const int L = zstar_lines<T>(N2, g.m);
   const size_t lds = (size_t)g.m * L * (N2 + 1) * sizeof(cpx<T>);
-  dim3 grid(B, (NH + L - 1) / L, N1);
+  const dim3 grid = g.zorder ? dim3((NH + L - 1) / L, N1, B) : dim3(B, (NH + L - 1) / L, N1);
   switch (L) {
     case 2: launch_zstar_l<T, 2>(N2, grid, lds, s, sd, g, tw, chi, tau); break;
     case 4: launch_zstar_l<T, 4>(N2, grid, lds, s, sd, g, tw, chi, tau); break;
@@ -760,6 +768,7 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   g.tau_scale = pd.beta * EPSILON0INV / L3;
   g.beta = pd.beta;
   g.renorm = in.opt.renorm ? 1 : 0;
+  g.zorder = env_int("RISM3D_ZSTAR_ORDER", 1) ? 1 : 0;
   g.closure = pd.closure;
   g.pse_order = pd.closure == RISM3D_CLOSURE_KH ? 1 : (pd.closure >= RISM3D_CLOSURE_PSE0 ? pd.closure - RISM3D_CLOSURE_PSE0 : 0);
   for (int i = 0; i < in.m; ++i) g.q[i] = in.solvent[i].charge;
