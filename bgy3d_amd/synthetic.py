@@ -19,7 +19,9 @@ import math
 import numpy as np
 
 
-def solute(seed, n=None):
+def solute(seed, n=None, net=None):
+    """Sites of the solute of this seed; n and the net charge default to the
+    draws described above."""
     rng = np.random.default_rng(seed)
     if n is None:
         n = int(rng.integers(5, 31))
@@ -38,7 +40,7 @@ def solute(seed, n=None):
     sigma = rng.uniform(2.5, 4.0, n)
     eps = rng.uniform(0.05, 0.25, n)
     q = rng.normal(0.0, 0.4, n)
-    Q = seed % 3 - 1
+    Q = seed % 3 - 1 if net is None else net
     q += (Q - q.sum()) / n
     return tuple((f"S{i}", tuple(map(float, x[i])), float(sigma[i]), float(eps[i]), float(q[i]))
                  for i in range(n))

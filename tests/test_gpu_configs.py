@@ -1,0 +1,87 @@
+"""The larger benchmark configurations as GPU parity cases.
+
+Config 2 (precision study): one 120-site synthetic solute, net charge -2,
+PSE-3, --L 32 --N 256 (64 A box, h = 0.25 A), PR-SPC/E water.  |dmu| and
+max|dg| of FP32-only and FP32 + 3 FP64 polish steps against the FP64 solve;
+bar |dmu| < 0.01 kcal/mol.  There is no published mu for this case (parity
+of the FP64 path to the oracle is covered on smaller grids).
+
+Config 3 (gradients): one 30-site solute, HNC, --L 20 --N 128; adjoint
+d mu / d theta against central finite differences of the GPU mu (h = 1e-4)
+to 1e-4 relative, and the time of forward + adjoint against forward alone.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+W = "water, PR-SPC/E"
+BASE = {"rho": 0.0333295, "beta": 1.6889, "rmax": 160.0, "nrad": 4096, "max-iter": 3000}
+
+
+@pytest.fixture(scope="module")
+def chi160():
+    from conftest import load_chi
+    return load_chi("chi_prspce_hnc_eps78.4_rmax160_nrad4096.npy")
+
+
+def test_config2_precision_study(chi160):
+    print("case config2 256^3 PSE3 120 sites Q=-2", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd import synthetic as S
+    sol = S.solute(2, n=120, net=-2)
+    s = dict(BASE, N=256, L=32.0, closure="PSE3", **{"norm-tol": 1e-9})
+    t0 = time.time()
+    f64 = R.hnc3d_run_solute(sol, W, s, chi=chi160)
+    t64 = time.time() - t0
+    assert f64["status"] == "OK"
+    rows = []
+    for name, polish in (("fp32-only", 0), ("fp32+3xfp64", 3)):
+        t0 = time.time()
+        m = R.hnc3d_run_solute(sol, W, dict(s, precision="mixed", **{"polish-iters": polish,
+                                                                      "mixed-tol": 1e-5}),
+                               chi=chi160)
+        dt = time.time() - t0
+        dmu = abs(m["free-energy"] - f64["free-energy"])
+        dg = float(np.max(np.abs(m["GUV"] - f64["GUV"])))
+        rows.append((name, dmu, dg))
+        print(f"  {name}: mu {m['free-energy']:.6f} |dmu| {dmu:.2e} max|dg| {dg:.2e} "
+              f"iters {m['iterations']} (fp32 {m['iterations-fp32']}) {dt:.2f} s", flush=True)
+        assert dmu < 0.01
+    print(f"  fp64: mu {f64['free-energy']:.6f} iters {f64['iterations']} {t64:.2f} s", flush=True)
+
+
+def test_config3_adjoint(chi160):
+    print("case config3 128^3 HNC 30 sites adjoint", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd import synthetic as S
+    sol = list(S.solute(3, n=30))
+    s = dict(BASE, N=128, L=20.0, closure="HNC", **{"norm-tol": 1e-11})
+    t0 = time.time()
+    fwd = R.hnc3d_run_solute(sol, W, s, chi=chi160)
+    t_fwd = time.time() - t0
+    t0 = time.time()
+    out = R.hnc3d_run_solute(sol, W, s, chi=chi160, adjoint=True, derivatives=True)
+    t_both = time.time() - t0
+    g = out["free-energy-dsite"]
+    assert np.all(np.isfinite(g))
+    print(f"  forward {t_fwd:.2f} s, forward + adjoint {t_both:.2f} s "
+          f"(adjoint iters {out['adjoint-iterations']})", flush=True)
+    h = 1e-4
+    for a, p in ((0, 0), (7, 2), (15, 3), (22, 4), (29, 5)):
+        def moved(sign):
+            x = [[n, list(xyz), sg, ep, q] for n, xyz, sg, ep, q in sol]
+            if p < 3:
+                x[a][1][p] += sign * h
+            elif p == 3:
+                x[a][4] += sign * h
+            elif p == 4:
+                x[a][2] += sign * h
+            else:
+                x[a][3] += sign * h
+            return [(n, tuple(xyz), sg, ep, q) for n, xyz, sg, ep, q in x]
+        fd = (R.hnc3d_run_solute(moved(+1), W, s, chi=chi160)["free-energy"]
+              - R.hnc3d_run_solute(moved(-1), W, s, chi=chi160)["free-energy"]) / (2 * h)
+        print(f"  site {a} param {p}: adjoint {g[a, p]:.8f} fd {fd:.8f}", flush=True)
+        assert abs(g[a, p] - fd) <= 1e-4 * max(abs(fd), 1e-2)
