@@ -697,6 +697,9 @@ __device__ __forceinline__ double form1_mu(int method, int order, double x, doub
 // mode 1 (adjoint, prep): D_i = dc/dt, gmu_i = dmu/dt.
 // mode 2 (adjoint, final): A_i = beta (K lambda)_i dc/d(beta v) + dmu/dv_i,
 //                          B = rho h^3 / 2 sum_i q_i h_i  (dmu/du_c).
+// mode 4 (linear response, hnc3d.c:1969-2134): as mode 2 with the
+//   reference's dc/d(beta v) = -(dc/dt + 1) (its Jacobian flag hack,
+//   hnc3d.c:1310-1320) for every closure; B unused.
 __global__ void __launch_bounds__(NT)
 adj_coef(GridArgs g, SitesDesc sd, const double* __restrict__ t, const double* __restrict__ bv,
          const double* __restrict__ Klam, double L0, double L1, double L2, double h0, double h1,
@@ -736,8 +739,8 @@ adj_coef(GridArgs g, SitesDesc sd, const double* __restrict__ t, const double* _
         D[q] = ct;
         gmu[q] = fmu * form1_mu(g.closure, g.pse_order, x, 1.0, h, ct + 1, c, ct, cl);
       } else {
-        const double dcv = cv * g.beta;
-        A[q] = g.beta * Klam[q] * cv +
+        const double dcv = (mode == 4 ? -(ct + 1) : cv) * g.beta;
+        A[q] = Klam[q] * dcv +
                fmu * form1_mu(g.closure, g.pse_order, x, -g.beta, h, dcv, c, dcv, cl);
         bsum += g.q[i] * h;
       }

@@ -580,11 +580,16 @@ int validate(const Inputs& in) {
 
 
 // Gradients after a converged solve (t in t64 of every solute).
-// TPT ('free-energy-gradient, hnc3d.c:1666-1742) for res[b].gradient, and
-// the exact adjoint d mu / d(x, y, z, q, sigma, epsilon) for
-// res[b].dmu_dsite: lambda = D_t K lambda + dmu/dt by linear MDIIS, then
-// K lambda, Lambda = r2c(lambda), the coefficient fields and two
-// reductions (grad_field, grad_tau).
+// TPT ('free-energy-gradient, hnc3d.c:1666-1742) for res[b].gradient, the
+// exact adjoint d mu / d(x, y, z, q, sigma, epsilon) for res[b].dmu_dsite,
+// and the reference's linear-response gradient ('free-energy-response,
+// hnc3d.c:1969-2134) for res[b].response.  The last two share one linear
+// solve: lambda = D_t K lambda + dmu/dt by linear MDIIS, then K lambda.
+// The reference's response path solves J dt = beta J' dv once per site
+// and axis (3 n solves); by J^T = D_t K - 1 (K symmetric) its result is
+// sum_i <A_i | dv_i> with A_i = -beta (D_i + 1) (K lambda)_i + dmu/dv_i,
+// one field reduction.  The adjoint adds Lambda = r2c(lambda) and the
+// tau and u_c terms (grad_tau, grad_field mode 1).
 void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& cf0,
                const std::vector<SitesDesc>& sites) {
   const GridArgs& g = pb.g;
@@ -597,12 +602,22 @@ void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& 
     sse[3 * i + 1] = in.solvent[i].epsilon;
     sse[3 * i + 2] = in.solvent[i].charge;
   }
-  bool want_tpt = false, want_adj = false;
+  bool want_tpt = false, want_lam = false;
+  // the reference's closure1 has no PY branch (closures.f90:131-153), so
+  // it has no response for PY; the adjoint is built for the tau path
+  auto wants_adj = [&](int b) { return res[b].dmu_dsite != nullptr && g.renorm; };
+  auto wants_resp = [&](int b) { return res[b].response != nullptr && g.closure != 2; };
   for (int b = 0; b < B; ++b) {
-    want_tpt |= res[b].gradient != nullptr && pb.st[b].status == RISM3D_OK;
-    want_adj |= res[b].dmu_dsite != nullptr && pb.st[b].status == RISM3D_OK;
+    const int n = sites[b].n;
+    if (res[b].dmu_dsite && !wants_adj(b))
+      for (int q = 0; q < n * 6; ++q) res[b].dmu_dsite[q] = NAN;
+    if (res[b].response && !wants_resp(b))
+      for (int q = 0; q < n * 3; ++q) res[b].response[q] = NAN;
+    if (pb.st[b].status != RISM3D_OK) continue;
+    want_tpt |= res[b].gradient != nullptr;
+    want_lam |= wants_adj(b) || wants_resp(b);
   }
-  if (!want_tpt && !want_adj) return;
+  if (!want_tpt && !want_lam) return;
   DevBuf<double> d_sse(sse.size());
   HIPCHK(hipMemcpyAsync(d_sse.p, sse.data(), sse.size() * 8, hipMemcpyHostToDevice, pb.s));
   const int cp = (int)std::min<int64_t>(512, (g.P + NT - 1) / NT);
@@ -636,18 +651,12 @@ void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& 
         for (int d = 0; d < 3; ++d) res[b].gradient[a * 3 + d] = acc[a * 6 + d];
     }
   }
-  if (!want_adj) return;
-  if (!g.renorm) {
-    for (int b = 0; b < B; ++b)
-      if (res[b].dmu_dsite)
-        for (int q = 0; q < sites[b].n * 6; ++q) res[b].dmu_dsite[q] = NAN;
-    return;
-  }
+  if (!want_lam) return;
   // adjoint solve for every solute that wants it, batched
   std::vector<DevBuf<double>> tsave(B);
   for (int b = 0; b < B; ++b) {
     auto& s = pb.st[b];
-    const bool want = res[b].dmu_dsite && s.status == RISM3D_OK;
+    const bool want = (wants_adj(b) || wants_resp(b)) && s.status == RISM3D_OK;
     s.active = want;
     if (!want) continue;
     s.Dt = DevBuf<double>(mP);
@@ -676,7 +685,12 @@ void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& 
     res[b].adjoint_iterations = s.iters;
     const int n = sites[b].n;
     if (s.status != RISM3D_OK) {
-      for (int q = 0; q < n * 6; ++q) res[b].dmu_dsite[q] = NAN;
+      if (wants_adj(b))
+        for (int q = 0; q < n * 6; ++q) res[b].dmu_dsite[q] = NAN;
+      if (wants_resp(b))
+        for (int q = 0; q < n * 3; ++q) res[b].response[q] = NAN;
+      HIPCHK(hipMemcpyAsync(s.t64.p, tsave[b].p, mP * 8, hipMemcpyDeviceToDevice, pb.s));
+      s.status = RISM3D_OK;
       continue;
     }
     // K lambda (mode 2) into R, then Lambda = r2c(lambda) (mode 3) into W
@@ -693,6 +707,22 @@ void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& 
     d.part = s.part.p;
     d.mode = 2;
     evaluate(pb, false, sd);
+    if (wants_resp(b)) {
+      hipLaunchKernelGGL(adj_coef, dim3(cp), dim3(NT), 0, pb.s, g, sites[b], tsave[b].p,
+                         s.bv64.p, (const double*)s.R.p, pd.L[0], pd.L[1], pd.L[2], pb.h[0],
+                         pb.h[1], pb.h[2], pd.rho, 4, A.p, Bf.p, (double*)nullptr,
+                         (double*)nullptr);
+      std::vector<double> acc;
+      field_reduce(b, 0, acc);
+      for (int a = 0; a < n; ++a)
+        for (int q = 0; q < 3; ++q) res[b].response[a * 3 + q] = acc[a * 6 + q];
+    }
+    if (!wants_adj(b)) {
+      HIPCHK(hipMemcpyAsync(s.t64.p, tsave[b].p, mP * 8, hipMemcpyDeviceToDevice, pb.s));
+      HIPCHK(hipStreamSynchronize(pb.s));
+      s.status = RISM3D_OK;
+      continue;
+    }
     d.mode = 3;
     {
       // forward passes only: closure_r2c (identity), fft_y, zstar (forward)

@@ -76,7 +76,8 @@ class Result(ctypes.Structure):
                 ("t", ctypes.POINTER(ctypes.c_double)),
                 ("gradient", ctypes.POINTER(ctypes.c_double)),
                 ("dmu_dsite", ctypes.POINTER(ctypes.c_double)),
-                ("adjoint_iterations", ctypes.c_int)]
+                ("adjoint_iterations", ctypes.c_int),
+                ("response", ctypes.POINTER(ctypes.c_double))]
 
 
 EXPORTS = ("rism3d_options_default", "rism3d_solute_solve", "rism3d_batch_solve",
@@ -205,7 +206,7 @@ def _dptr(a):
 
 
 def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=False,
-                     derivatives=False, adjoint=False):
+                     derivatives=False, adjoint=False, response=False):
     """Solve the 3D-RISM solute problem on the GPU.
 
     Returns a dict like the reference alist: 'free-energy',
@@ -213,9 +214,12 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
     'free-energy-hnc'/'-kh'/'-gf', 'iterations', 'residual', 'RESTART'
     (opaque token, pass back as ``restart=`` and free with
     ``restart_destroy``) if ``restart`` is not None.
-    derivatives=True adds 'free-energy-gradient' (n, 3), the reference's
-    TPT gradient (--derivatives); adjoint=True adds 'free-energy-dsite'
-    (n, 6), exact d mu / d(x, y, z, q, sigma, epsilon) by the adjoint."""
+    derivatives=True (or the 'derivatives setting, as hnc3d.c:1783 reads it)
+    adds 'free-energy-gradient' (n, 3), the reference's TPT gradient;
+    response=True (or the 'response setting, hnc3d.c:1784) adds
+    'free-energy-response' (n, 3), its linear-response gradient;
+    adjoint=True adds 'free-energy-dsite' (n, 6), exact
+    d mu / d(x, y, z, q, sigma, epsilon) by the adjoint."""
     lib = load()
     pd = problem_data(settings)
     opt = make_options(settings)
@@ -230,7 +234,12 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
     if want_t:
         t = np.empty(shape)
         res.t = _dptr(t)
-    grad = dsite = None
+    derivatives = derivatives or bool(settings.get("derivatives", False))
+    response = response or bool(settings.get("response", False))
+    grad = dsite = resp = None
+    if response:
+        resp = np.full((len(su), 3), np.nan)
+        res.response = _dptr(resp)
     if derivatives:
         grad = np.full((len(su), 3), np.nan)
         res.gradient = _dptr(grad)
@@ -262,6 +271,8 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
         out["t"] = t
     if grad is not None:
         out["free-energy-gradient"] = grad
+    if resp is not None:
+        out["free-energy-response"] = resp
     if dsite is not None:
         out["free-energy-dsite"] = dsite
         out["adjoint-iterations"] = res.adjoint_iterations

@@ -133,6 +133,16 @@ typedef struct rism3d_result {
                           adjoint of the residual (renorm path only; no
                           counterpart in the reference)                     */
   int adjoint_iterations;  /* linear MDIIS steps of the adjoint solve      */
+  double *response;    /* in: NULL or [n][3]; out: the reference's linear-
+                          response gradient ('free-energy-response,
+                          --response; hnc3d.c:1366-1421, 1969-2134): the
+                          first order change of mu along each site's x, y,
+                          z through v_short at fixed u_c and tau, with the
+                          full capped Coulomb derivative in dv.  Equal to
+                          the x, y, z of dmu_dsite when the solute is
+                          uncharged.  Either path; NaN for PY, whose
+                          differential the reference lacks
+                          (closures.f90:131-153)                          */
 } rism3d_result;
 
 /* Opaque restart token: the converged t (hnc3d.c:1826-1845, 2299-2316). */

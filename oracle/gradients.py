@@ -9,6 +9,16 @@ TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``.
   Coulomb derivative ccap1 (bgy3d-solutes.c:102-131, 290-305).
   Pinned: dv/dx against central differences of the full site potential.
 
+* ``response_gradient`` restates the reference's --response path
+  (hnc3d.c:1366-1421 response_t1, hnc3d.c:1969-2134): for every solute
+  site and axis, dv = field1 (full capped Coulomb), b = beta J' dv with
+  the Jacobian's flag hack (hnc3d.c:1310-1320), J dt = b solved to a
+  relative tolerance, then chempot1 (closures.f90:615-653) of dx = -beta dv
+  + dt, dc = c1(dx) - beta dv, dh = dc + dt at fixed cl.  3 n linear
+  solves.  Pinned: equal to central differences of mu for an uncharged
+  solute (the long-range terms it leaves out vanish there), and to the
+  one-solve adjoint form sum_i <A_i | dv_i> for charged ones.
+
 * ``adjoint_gradient`` is exact d mu / d(x, y, z, q, sigma, epsilon) of
   every solute site on the renormalised path, by the adjoint of the
   residual: with J = dF/dt = K D_t - 1 (K the chi convolution, symmetric
@@ -86,6 +96,21 @@ def full_field(pd, a, solute):
     return v
 
 
+def field1(pd, a, b, d):
+    """d v_a / d x_b[d] of solvent site a in the field of solute site b
+    (bgy3d-solutes.c:381-468 with one moving site): capped LJ and capped
+    full Coulomb derivatives times d r / d x_b."""
+    dx, dy, dz = _rvec(pd, b[1])
+    r = np.sqrt(dx * dx + dy * dy + dz * dz)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = np.where(r > 0, 1.0 / r, 0.0)
+    eab = math.sqrt(a[3] * b[3])
+    sab = 0.5 * (a[2] + b[2])
+    qab = a[4] * b[4] * EPS0INV
+    f = (eab / sab) * ljcap1(r / sab) + qab * G * G * ccap1(G * r)
+    return f * (-(dx, dy, dz)[d] * inv)
+
+
 def tpt_gradient(pd, solvent, solute, g):
     """gradients() of hnc3d.c:1726-1742: de[n][3]."""
     dN = pd.rho * pd.h3
@@ -103,6 +128,76 @@ def tpt_gradient(pd, solvent, solute, g):
             for d, comp in enumerate((dx, dy, dz)):
                 de[a, d] += dN * np.sum(g[i] * f * (-comp * inv))
     return de
+
+
+def _mu_fields(sol, t):
+    pd = sol.pd
+    beta = pd.beta
+    c = C.closure(pd.closure, beta, sol.v, t)
+    h = c + t
+    x = -beta * sol.v + t
+    uc = O.coulomb_long(pd, sol.solute)
+    cl = -beta * sol.q[:, None, None, None] * uc[None]
+    return x, h, c, cl
+
+
+def response_gradient(sol, t, rel_tol=1e-12):
+    """'free-energy-response of hnc3d.c:1969-2134: dE[n][3]."""
+    pd = sol.pd
+    beta = pd.beta
+    if pd.closure == C.PY:
+        raise ValueError("the reference's closure1 has no PY branch")
+    fmu = pd.rho * pd.h3 / beta
+    x, h, c, cl = _mu_fields(sol, t)
+    dE = np.zeros((len(sol.solute), 3))
+    for a, b in enumerate(sol.solute):
+        for d in range(3):
+            dv = np.stack([field1(pd, s, b, d) for s in sol.solvent])
+            rhs = beta * sol.jacobian(t, dv, flag=True)
+            atol = rel_tol * max(np.max(np.abs(rhs)), 1e-300)
+            dt, _ = mdiis(lambda y: sol.jacobian(t, y) - rhs, np.zeros_like(t),
+                          atol=atol, max_iter=5000)
+            dx = -beta * dv + dt
+            dc = C.closure1(pd.closure, beta, sol.v, t, dx) - beta * dv
+            dh = dc + dt
+            dE[a, d] = fmu * math.fsum(C.form1(pd.closure, x, dx, h, dh, c, dc, cl).ravel())
+    return dE
+
+
+def response_from_adjoint(sol, t, lam):
+    """The same numbers from one adjoint solve: with lambda = D_t K lambda +
+    dmu/dt, dE[a][d] = sum_i <A_i | field1_i>, A_i = -beta (ct + 1) K lambda
+    + dmu/dv at fixed t (ct = dc/dt)."""
+    pd = sol.pd
+    beta = pd.beta
+    fmu = pd.rho * pd.h3 / beta
+    x, h, c, cl = _mu_fields(sol, t)
+    ct = C.closure1(pd.closure, beta, sol.v, t, np.ones_like(t))
+    Klam = O.irfft(pd, sol.star(O.rfft(pd, lam) * pd.h3)) / pd.L3
+    dcv = -beta * (ct + 1)
+    A = Klam * dcv + fmu * C.form1(pd.closure, x, -beta, h, dcv, c, dcv, cl)
+    dE = np.zeros((len(sol.solute), 3))
+    for a, b in enumerate(sol.solute):
+        for d in range(3):
+            dE[a, d] = sum(np.sum(A[i] * field1(pd, s, b, d)) for i, s in enumerate(sol.solvent))
+    return dE
+
+
+def adjoint_lambda(sol, t, rel_tol=1e-12):
+    """lambda = D_t K lambda + dmu/dt (any closure but PY, either path)."""
+    pd = sol.pd
+    fmu = pd.rho * pd.h3 / pd.beta
+    x, h, c, cl = _mu_fields(sol, t)
+    one = np.ones_like(t)
+    ct = C.closure1(pd.closure, pd.beta, sol.v, t, one)
+    gmu = fmu * C.form1(pd.closure, x, one, h, ct + 1, c, ct, cl)
+
+    def K(v):
+        return O.irfft(pd, sol.star(O.rfft(pd, v) * pd.h3)) / pd.L3
+
+    lam, _ = mdiis(lambda l: ct * K(l) + gmu - l, np.zeros_like(t),
+                   atol=rel_tol * np.max(np.abs(gmu)), max_iter=5000)
+    return lam
 
 
 def _kweights(pd):
@@ -153,7 +248,7 @@ def adjoint_gradient(sol, t, lam_rel_tol=1e-12):
     # tau term: -tau_scale Re sum_k w tau_i s dF/dtheta conj(Lambda_i)
     Lam = O.rfft(pd, lam)
     s = O.phase_sign(pd)
-    tau0 = sol.tau / (O.form_factor(pd, sol.solute) * s)[None]     # tau_i(|k|)
+    tau0 = sol.tau_k
     S = np.sum(tau0 * np.conj(Lam), axis=0) * s * _kweights(pd)
     bscale = beta * EPS0INV / pd.L3
     kx, ky, kz = O.k_axes(pd)

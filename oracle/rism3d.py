@@ -257,6 +257,7 @@ class Solute3D:
         self.m = len(solvent)
         self.q = np.array([s[4] for s in solvent])
         self.chi, tau = solvent_kernel_rism(pd, solvent, chi_1d)
+        self.tau_k = tau                    # tau_i(|k|) before F(k) s(k)
         self.renorm = renorm
         if renorm:
             s = phase_sign(pd)

@@ -9,6 +9,10 @@
   tests/test_oracle_gradients.py) to 1e-6 relative, and one direct central
   finite difference of the GPU mu (h = 1e-4) to 1e-4 relative, the bar of
   the north star.
+* Linear-response gradient (the reference's --response, hnc3d.c:1969-2134),
+  computed on the GPU from one adjoint solve, vs the oracle's literal
+  restatement with 3 n linear solves: 1e-6 relative, on both the tau and
+  the Ng path, and for an uncharged solute vs the adjoint's x, y, z.
 """
 import numpy as np
 import pytest
@@ -83,3 +87,37 @@ def test_adjoint_vs_finite_difference(chi_w):
         adj = out["free-energy-dsite"][a, p]
         print(f"  site {a} param {p}: adjoint {adj:.8f} fd {fd:.8f}", flush=True)
         assert abs(adj - fd) <= 1e-4 * max(abs(fd), 1e-2)
+
+
+@pytest.mark.parametrize("closure,renorm", [("KH", True), ("HNC", False)])
+def test_response_vs_oracle(closure, renorm, chi_w):
+    print(f"case response {closure} renorm={renorm}", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd.solutes import find_molecule
+    sol = find_molecule("water, cSPC/E")
+    s = _settings(closure)
+    if not renorm:
+        s["no-renorm"] = True
+    out = R.hnc3d_run_solute(sol, W, dict(s, response=True), chi=chi_w)
+    pd = O.Problem.from_cli(8.0, 32, README["beta"], README["rho"], closure=C.NAMES[closure],
+                            rmax=40.0, nrad=1536)
+    ref = O.Solute3D(pd, find_molecule(W), sol, chi_w, renorm=renorm)
+    t, _ = ref.solve(atol=1e-12)
+    lit = Gr.response_gradient(ref, t)
+    e = np.max(np.abs(out["free-energy-response"] - lit)) / np.max(np.abs(lit))
+    print(f"  response rel {e:.2e}  adj iters {out['adjoint-iterations']}", flush=True)
+    assert e < 1e-6
+
+
+def test_response_uncharged_equals_adjoint(chi_w):
+    print("case response uncharged", flush=True)
+    from bgy3d_amd import rism3d as R
+    sol = [("A", (0.0, 0.0, 0.9), 3.16, 0.155, 0.0), ("B", (0.3, 0.1, -0.9), 3.0, 0.1, 0.0)]
+    out = R.hnc3d_run_solute(sol, W, _settings("HNC"), chi=chi_w, response=True, adjoint=True,
+                             derivatives=True)
+    r, d = out["free-energy-response"], out["free-energy-dsite"][:, :3]
+    e = np.max(np.abs(r - d)) / np.max(np.abs(d))
+    print(f"  response vs adjoint rel {e:.2e}", flush=True)
+    assert e < 1e-9
+    # the TPT gradient is a different (first-order perturbation) quantity
+    assert np.max(np.abs(out["free-energy-gradient"] - d)) > 1e-3 * np.max(np.abs(d))
