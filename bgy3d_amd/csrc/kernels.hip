@@ -14,39 +14,11 @@
 // bgy3d-fftw3.c:364-385.
 #include "fft_device.hpp"
 #include "kernels.hpp"
+#include "closure_math.hpp"
 
 namespace rism {
 
-// ---------------------------------------------------------------- closures
-template <typename T> __device__ __forceinline__ T d_expm1(T x);
-template <> __device__ __forceinline__ double d_expm1<double>(double x) { return expm1(x); }
-template <> __device__ __forceinline__ float d_expm1<float>(float x) { return expm1f(x); }
-template <typename T> __device__ __forceinline__ T d_exp(T x);
-template <> __device__ __forceinline__ double d_exp<double>(double x) { return exp(x); }
-template <> __device__ __forceinline__ float d_exp<float>(float x) { return expf(x); }
-
-// Truncated exponential series sum_{k=1..n} x^k/k! by Horner
-// (closures.f90:191-236).
-template <typename T> __device__ __forceinline__ T exps1(int n, T x) {
-  T c = T(1), y = T(0);
-  for (int k = n; k >= 1; --k) { y = x * y + c; c = c * T(k); }
-  return (x / c) * y;
-}
-
-// c = closure(beta v, t) (closures.f90:107-128).  bv = beta * v.
-template <typename T> __device__ __forceinline__ T closure_c(int method, int order, T bv, T t) {
-  const T x = -bv + t;
-  switch (method) {
-    case 0:  // HNC
-      return d_expm1(x) - t;
-    case 1:  // KH
-      return (x <= T(0)) ? d_expm1(x) - t : -bv;
-    case 2:  // PY
-      return d_exp(-bv) * (T(1) + t) - T(1) - t;
-    default:  // PSE-n
-      return ((x < T(0)) ? d_expm1(x) : exps1(order, x)) - t;
-  }
-}
+// closure_c, closure_dc and their helpers: closure_math.hpp
 
 // Chemical-potential density form(x, h, c, cl) (closures.f90:574-612).
 __device__ __forceinline__ double form_mu(int method, int order, double x, double h, double c,
@@ -100,12 +72,16 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
       if (nc > 0) {
         double s = 0.0;
         for (int j = 0; j < nc; ++j) s += d.coef[j] * (double)static_cast<const T*>(d.u[j])[off + e];
+        if (d.shift_w != 0.0) s -= d.shift_w * (double)static_cast<const T*>(d.shift)[off + e];
         tv = T(s);
         t[e] = tv;
       } else {
         tv = t[e];
       }
-      c = (d.mode == 0) ? closure_c<T>(g.closure, g.pse_order, bv[e], tv) : tv;
+      if (d.mode == 0) c = closure_c<T>(g.closure, g.pse_order, bv[e], tv);
+      else if (d.mode == 5)  // bv holds x_b (PY: exp(-beta v) - 1, c linear in t)
+        c = (g.closure == 2) ? bv[e] * tv : closure_dc<T>(g.closure, g.pse_order, bv[e], tv);
+      else c = tv;
     }
     ldsr[e] = c;  // z_n = c_{2n} + i c_{2n+1}, row stride M complex
   }
@@ -372,7 +348,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   const int64_t off = site * g.P + (int64_t)row0 * N0;
   const T* __restrict__ t = static_cast<const T*>(d.t) + off;
   T* __restrict__ r = static_cast<T*>(d.r) + off;
-  T* __restrict__ un = static_cast<T*>(d.unew) + off;
+  T* __restrict__ un = d.unew ? static_cast<T*>(d.unew) + off : nullptr;  // null: no u
   const int nd = d.ndot;
   const T lam = T(d.lam);
   double mx = 0.0, ss = 0.0;
@@ -384,10 +360,11 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
     const T tv = t[e];
     T rv;
     if (mode == 0) rv = zr[e] - tv;                                    // F(t)
+    else if (mode == 5) rv = static_cast<const T*>(d.fb)[off + e] + zr[e] - tv;  // defect
     else if (mode == 1) rv = T(d.dt[off + e]) * zr[e] + T(d.gv[off + e]) - tv;  // adjoint
     else rv = zr[e];                                                   // K t
     r[e] = rv;
-    un[e] = tv + lam * rv;
+    if (un) un[e] = tv + lam * rv;
     const double rd = (double)rv;
     mx = fmax(mx, fabs(rd));
     ss += rd * rd;
@@ -464,6 +441,46 @@ convert_batch(void* const* __restrict__ dst, const void* const* __restrict__ src
       static_cast<double*>(dst[blockIdx.y])[e] = (double)static_cast<const float*>(src[blockIdx.y])[e];
     else
       static_cast<float*>(dst[blockIdx.y])[e] = (float)static_cast<const double*>(src[blockIdx.y])[e];
+  }
+}
+
+// u = t + lam r, FP64, for one solute (first MDIIS entry of an FP64 phase).
+__global__ void __launch_bounds__(NT)
+seed_u(double* __restrict__ u, const double* __restrict__ t, const double* __restrict__ r,
+       double lam, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride)
+    u[e] = t[e] + lam * r[e];
+}
+
+// Defect correction, base update: t_b += delta for one solute.
+__global__ void __launch_bounds__(NT)
+add_delta(double* __restrict__ tb, const float* __restrict__ delta, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride)
+    tb[e] += (double)delta[e];
+}
+
+// Defect correction, new base t_b with F(t_b) = f64 known: x_b = -beta v +
+// t_b (PY: exp(-beta v) - 1) into xb, F(t_b) into fb; with seed (the first
+// base of a solve) also delta = 0 and the first MDIIS entry r_0 = F(t_b),
+// u_0 = 0 + lam r_0.
+__global__ void __launch_bounds__(NT)
+rebase_prep(const double* __restrict__ tb, const double* __restrict__ bv,
+            const double* __restrict__ f64, int py, int seed, float lam,
+            float* __restrict__ xb, float* __restrict__ fb, float* __restrict__ delta,
+            float* __restrict__ u0, float* __restrict__ r0, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) {
+    const double b = bv[e];
+    xb[e] = py ? (float)expm1(-b) : (float)(-b + tb[e]);
+    const float f = (float)f64[e];
+    fb[e] = f;
+    if (seed) {
+      delta[e] = 0.f;
+      r0[e] = f;
+      u0[e] = lam * f;
+    }
   }
 }
 

@@ -31,9 +31,16 @@ struct SoluteDesc {
   const void* Fs;            // [N2][N1][NH] form factor times s(k), cpx<T>
   double* part;              // per residual workgroup: max|r|, sum r^2, dots
   int active;                // 0: converged / failed / waiting: return
-  int mode;                  // 0 F(t); 1 adjoint D K l + g - l; 2 K l; 3 spectrum
+  int mode;                  // 0 F(t); 1 adjoint D K l + g - l; 2 K l; 3 spectrum;
+                             // 5 defect correction: F(t_b + delta) for the
+                             //   correction delta (t) around a base t_b, as
+                             //   F(t_b) + K (c(t_b + delta) - c(t_b)) - delta
   const double* dt;          // mode 1: D = dc/dt at the converged t (FP64)
   const double* gv;          // mode 1: g = dmu/dt
+  const void* fb;            // mode 5: F(t_b); bv then holds x_b = -beta v + t_b
+                             //   (PY: exp(-beta v) - 1)
+  const void* shift;         // mode 5: t = sum_j coef_j u_j - shift_w shift: the
+  double shift_w;            //   entries of the previous base moved to this one
 };
 constexpr int NPART = 2 + MAX_DIIS;   // doubles per residual workgroup
 

@@ -227,6 +227,14 @@ struct SoluteState {
   DevBuf<double> part;     // residual-workgroup partials, NPART each
   DevBuf<double> sites;    // n * 6
   DevBuf<double> Dt, gmu;  // adjoint: dc/dt and dmu/dt at the converged t
+  DevBuf<double> r64;      // mixed: F(t_b) at a rebase, FP64
+  DevBuf<float> fb32;      // mixed: F(t_b) for the FP32 correction passes
+  DevBuf<float> t32b;      // mixed: second correction buffer
+  float* dcur = nullptr;   // mixed: the correction delta of the current base
+  float* dshift = nullptr; // mixed: the last delta folded into the base, by
+                           // which entries of the previous base are moved
+  int base = 0;            // mixed: rebases so far in this FP32 phase
+  std::vector<int> sbase;  // mixed: base of each history slot
   double tol_lin = 0.0;    // adjoint: absolute tolerance (relative to |g|)
   int n_sites = 0;
   // host MDIIS state
@@ -240,8 +248,9 @@ struct SoluteState {
   int iters = 0, iters32 = 0;
   int phase32 = 0;            // 1 while the FP32 kernels run
   int reached_mixed = 0;      // FP32 phase ended at mixed_tol or its floor
-  double best32 = INFINITY;
+  double best32 = INFINITY;   // FP32: best max|F| since the last rebase
   int stall32 = 0;
+  double rb_rinf = INFINITY;  // FP32: max|F(t_b)| at the last rebase (FP64)
   int polish = 0;             // FP64 steps taken after the FP32 phase
   double resid = INFINITY, best = INFINITY;
   int since_best = 0;
@@ -325,8 +334,8 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
       ++na;
       // u_j in, t out (or t in), beta v in, C out
       b0 += d.ncomb * mP * sc + mP * sc + mP * sc + mK * 2 * sc;
-      // W in, t in, r and u out, r_j in
-      b4 += mK * 2 * sc + mP * sc + 2 * mP * sc + d.ndot * mP * sc;
+      // W in, t in, r and u out, r_j in (+ F(t_b) in, defect correction)
+      b4 += mK * 2 * sc + mP * sc + 2 * mP * sc + d.ndot * mP * sc + (d.mode == 5 ? mP * sc : 0);
     }
     const double tables = (double)(g.m * (g.m + 1) / 2 + g.m) * g.K * sc;
     const double per[5] = {b0, na * 2 * mK * 2 * sc,
@@ -368,6 +377,104 @@ void convert_iterates(Problem& pb, const std::vector<int>& mask, bool to64) {
   HIPCHK(hipStreamSynchronize(pb.s));
 }
 
+// Defect correction (mixed precision).  The FP32 passes iterate on a
+// correction delta around an FP64 base t_b: F(t_b + delta) = F(t_b) +
+// K (c(t_b + delta) - c(t_b)) - delta, with F(t_b) from one FP64 residual
+// evaluation and the closure difference formed from x_b = -beta v + t_b in
+// a form whose FP32 rounding is relative to |delta c| (closure_dc).  The
+// FP32 error then scales with the size of the correction, not with |t|
+// and |c| (hundreds near charged sites, where a plain FP32 iteration stalls
+// at max|F| ~ 1e-4).  A rebase folds delta into t_b (FP64), evaluates
+// F(t_b) in FP64 -- the residual the stopping test uses -- and restarts
+// the FP32 MDIIS history with r_0 = F(t_b).  add = 0 on entry (delta = 0).
+// Returns, per solute of the mask, the FP64 max|F(t_b)| and ||F(t_b)||^2.
+void rebase(Problem& pb, const Config& cf, const std::vector<int>& mask, bool add,
+            std::vector<double>& rinf, std::vector<double>& r2) {
+  const int B = pb.B;
+  const int64_t mP = pb.mP;
+  const int ch = (int)std::min<int64_t>(2048, (mP + NT - 1) / NT);
+  std::vector<SoluteDesc> sd(B);
+  for (int b = 0; b < B; ++b) {
+    SoluteDesc& d = sd[b];
+    std::memset(&d, 0, sizeof d);
+    if (!mask[b]) continue;
+    auto& s = pb.st[b];
+    if (add) hipLaunchKernelGGL(add_delta, dim3(ch), dim3(NT), 0, pb.s, s.t64.p, s.dcur, mP);
+    d.active = 1;
+    d.t = s.t64.p;
+    d.r = s.r64.p;
+    d.unew = nullptr;  // the FP32 history lives in U: no u written
+    d.lam = cf.lam;
+    d.bv = s.bv64.p;
+    d.W = s.W.p;
+    d.Fs = s.fs64.p;
+    d.part = s.part.p;
+  }
+  evaluate(pb, false, sd);
+  std::vector<double> out((size_t)B * NPART);
+  HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, pb.s));
+  HIPCHK(hipStreamSynchronize(pb.s));
+  HIPCHK(hipGetLastError());
+  rinf.assign(B, 0.0);
+  r2.assign(B, 0.0);
+  for (int b = 0; b < B; ++b) {
+    if (!mask[b]) continue;
+    rinf[b] = out[(size_t)b * NPART];
+    r2[b] = out[(size_t)b * NPART + 1];
+  }
+}
+
+// New base after rebase(): x_b and F(t_b) for one solute.  seed (the first
+// base of the FP32 phase): delta = 0 and the first MDIIS entry, r_0 =
+// F(t_b), u_0 = lam r_0 in slot 0.  Later bases keep the MDIIS history: the
+// delta just folded into t_b becomes the shift of the entries of the old
+// base (t = sum_j c_j u_j - (sum_{old j} c_j) shift), entries of the base
+// before are dropped, and the coefficients are refitted if any was.
+void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
+  auto& s = pb.st[b];
+  const int64_t mP = pb.mP;
+  const int D = pb.D;
+  const int ch = (int)std::min<int64_t>(2048, (mP + NT - 1) / NT);
+  float* U0 = reinterpret_cast<float*>(s.U.p);
+  float* R0 = reinterpret_cast<float*>(s.R.p);
+  if (seed) {
+    s.dcur = s.t32.p;
+    s.dshift = s.t32b.p;
+    s.base = 0;
+    s.sbase.assign(D, 0);
+  }
+  hipLaunchKernelGGL(rebase_prep, dim3(ch), dim3(NT), 0, pb.s, s.t64.p, s.bv64.p, s.r64.p,
+                     pb.g.closure == RISM3D_CLOSURE_PY ? 1 : 0, seed ? 1 : 0, (float)cf.lam,
+                     s.bv32.p, s.fb32.p, s.dcur, U0, R0, mP);
+  s.best32 = INFINITY;
+  s.stall32 = 0;
+  if (seed) {
+    s.order.assign(1, 0);
+    s.coef.assign(1, 1.0);
+    s.gram.assign((size_t)D * D, 0.0);
+    s.gram[0] = r2;
+    s.best = r2;
+    s.since_best = 0;
+    return;
+  }
+  std::vector<int> keep;
+  for (int q : s.order)
+    if (s.sbase[q] == s.base) keep.push_back(q);
+  if (keep.size() != s.order.size()) {
+    s.order = keep;
+    const int nh = (int)s.order.size();
+    std::vector<double> G(nh * nh);
+    for (int i = 0; i < nh; ++i)
+      for (int j = 0; j < nh; ++j) G[i * nh + j] = s.gram[s.order[i] * D + s.order[j]];
+    if (nh == 1 || !diis_coefficients(G, nh, s.coef)) {
+      s.order.assign(1, s.order.back());
+      s.coef.assign(1, 1.0);
+    }
+  }
+  std::swap(s.dcur, s.dshift);
+  s.base++;
+}
+
 // Runs the fixed-point iteration for every solute of the batch.  MDIIS
 // (Pulay) over the history u_j = t_j + lam r_j: t = sum c_j u_j with c
 // minimising |sum c_j r_j|, sum c_j = 1.  Depth 1 is Picard, t += lam r
@@ -379,14 +486,71 @@ void iterate_batch(Problem& pb, const Config& cf) {
   const int D = pb.D;
   std::vector<SoluteDesc> sd(B);
   std::vector<double> out((size_t)B * NPART);
+  std::vector<int> act(B, 0);
   for (int b = 0; b < B; ++b) {
     auto& s = pb.st[b];
-    s.phase32 = cf.precision == RISM3D_MIXED;
+    act[b] = s.active != 0;
+    s.phase32 = s.active && cf.precision == RISM3D_MIXED;
     s.gram.assign(D * D, 0.0);
     s.order.clear();
     s.coef.clear();
   }
-  if (cf.precision == RISM3D_MIXED) convert_iterates(pb, std::vector<int>(B, 1), false);
+  // mixed: FP64 residual at the starting point, the first base
+  auto end_fp32 = [&](int b, double rinf, double r2) {
+    // the base passed the FP64 test at mixed_tol: polish or stop
+    auto& s = pb.st[b];
+    s.phase32 = 0;
+    s.reached_mixed = 1;
+    s.resid = rinf;
+    s.polish = 0;
+    if (rinf <= cf.tol || cf.polish == 0) {
+      s.status = RISM3D_OK;
+      s.active = 0;
+      return;
+    }
+    // FP64 history entry 0 from the rebase: r_0 = F(t_b), u_0 = t_b + lam r_0
+    HIPCHK(hipMemcpyAsync(s.R.p, s.r64.p, mP * 8, hipMemcpyDeviceToDevice, pb.s));
+    const int ch = (int)std::min<int64_t>(2048, (mP + NT - 1) / NT);
+    hipLaunchKernelGGL(seed_u, dim3(ch), dim3(NT), 0, pb.s, s.U.p, s.t64.p, s.r64.p, cf.lam, mP);
+    s.order.assign(1, 0);
+    s.coef.assign(1, 1.0);
+    s.gram.assign((size_t)D * D, 0.0);
+    s.gram[0] = r2;
+    s.best = r2;
+    s.since_best = 0;
+  };
+  auto after_rebase = [&](const std::vector<int>& mask, const std::vector<double>& rinf,
+                          const std::vector<double>& r2, bool seed) {
+    for (int b = 0; b < B; ++b) {
+      if (!mask[b]) continue;
+      auto& s = pb.st[b];
+      s.iters++;
+      s.resid = rinf[b];
+      if (cf.verbose)
+        std::fprintf(stderr, "rism3d: solute %d iter %d fp64 rebase |F|inf = %.3e\n", b, s.iters,
+                     rinf[b]);
+      if (!std::isfinite(rinf[b]) || !std::isfinite(r2[b])) {
+        s.status = RISM3D_NAN;
+        s.active = 0;
+        s.phase32 = 0;
+        continue;
+      }
+      if (rinf[b] <= cf.mixed_tol || rinf[b] <= cf.tol) { end_fp32(b, rinf[b], r2[b]); continue; }
+      if (s.iters32 >= cf.max_iter) {
+        s.status = RISM3D_NONCONVERGED;
+        s.active = 0;
+        s.phase32 = 0;
+        continue;
+      }
+      s.rb_rinf = rinf[b];
+      rebase_start(pb, cf, b, r2[b], seed);
+    }
+  };
+  if (cf.precision == RISM3D_MIXED) {
+    std::vector<double> rinf, r2;
+    rebase(pb, cf, act, false, rinf, r2);
+    after_rebase(act, rinf, r2, true);
+  }
   for (;;) {
     bool any = false, any32 = false;
     for (int b = 0; b < B; ++b) {
@@ -406,12 +570,14 @@ void iterate_batch(Problem& pb, const Config& cf) {
       if (!d.active) continue;
       auto slotU = [&](int q) { return (void*)((char*)s.U.p + (size_t)q * mP * sz); };
       auto slotR = [&](int q) { return (void*)((char*)s.R.p + (size_t)q * mP * sz); };
-      d.t = fp32 ? (void*)s.t32.p : (void*)s.t64.p;
+      d.t = fp32 ? (void*)s.dcur : (void*)s.t64.p;
       d.ncomb = (int)s.coef.size();
       for (int j = 0; j < d.ncomb; ++j) {
         d.u[j] = slotU(s.order[j]);
         d.coef[j] = s.coef[j];
+        if (fp32 && s.sbase[s.order[j]] != s.base) d.shift_w += s.coef[j];
       }
+      d.shift = s.dshift;
       // slot for the new r, u: a free one, else the oldest (dropped)
       s.dotted = s.order;
       if ((int)s.order.size() < D) {
@@ -431,17 +597,18 @@ void iterate_batch(Problem& pb, const Config& cf) {
       d.W = s.W.p;
       d.Fs = fp32 ? (const void*)s.fs32.p : (const void*)s.fs64.p;
       d.part = s.part.p;
-      d.mode = cf.mode;
+      d.mode = fp32 ? 5 : cf.mode;  // FP32 passes iterate the correction
       d.dt = s.Dt.p;
       d.gv = s.gmu.p;
+      d.fb = s.fb32.p;
     }
     evaluate(pb, fp32, sd);
     HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, pb.s));
     HIPCHK(hipStreamSynchronize(pb.s));
     HIPCHK(hipGetLastError());
 
-    std::vector<int> handover(B, 0);
-    bool any_handover = false;
+    std::vector<int> rebase_mask(B, 0);
+    bool any_rebase = false;
     for (int b = 0; b < B; ++b) {
       auto& s = pb.st[b];
       if (!sd[b].active) continue;
@@ -459,26 +626,19 @@ void iterate_batch(Problem& pb, const Config& cf) {
         continue;
       }
       if (fp32) {
-        // FP32 hands over at mixed_tol, or once its residual stops falling
-        // (the FP32 floor, ~1e-7 max|t|, can lie above mixed_tol near
-        // charged sites): no 30% gain on the best value in 12 iterations.
-        // Stalls count only near the floor (max|F| <= 1e-3): higher up a
-        // slow phase is not a floor, and handing over there would leave
-        // the FP64 polish far from the root.
+        // Rebase once the correction's residual fell 1e3 below F(t_b), or
+        // below mixed_tol / 2 (the FP64 test then decides), or stopped
+        // falling (no 30% gain on its best in 12 passes, counted below
+        // 1e-2 F(t_b)), or at the iteration limit.
         if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; }
-        else if (rinf <= 1e-3) { s.stall32++; }
-        if (rinf <= cf.mixed_tol || s.stall32 >= 12 || s.iters32 >= cf.max_iter) {
-          s.reached_mixed = rinf <= cf.mixed_tol || s.stall32 >= 12;
-          s.polish = 0;
-          s.phase32 = 0;
-          s.order.clear();
-          s.coef.clear();
-          s.best = INFINITY;
-          s.since_best = 0;
-          handover[b] = 1;
-          any_handover = true;
-          continue;  // the same iterate is evaluated again in FP64
+        else if (rinf <= 1e-2 * s.rb_rinf) { s.stall32++; }
+        // The new entry still enters the history (the rebase moves it).
+        if (rinf <= 1e-3 * s.rb_rinf || rinf <= 0.5 * cf.mixed_tol || s.stall32 >= 12 ||
+            s.iters32 >= cf.max_iter) {
+          rebase_mask[b] = 1;
+          any_rebase = true;
         }
+        s.sbase[s.slot_new] = s.base;
       } else {
         if (cf.mode == 1 && s.iters == 1) s.tol_lin = cf.rel_tol * rinf;  // r(0) = g
         // FP64 and the MIXED polish both aim at norm_tol; the polish stops
@@ -490,7 +650,10 @@ void iterate_batch(Problem& pb, const Config& cf) {
           continue;
         }
         if (cf.precision == RISM3D_MIXED && s.polish >= cf.polish) {
-          s.status = rinf <= cf.mixed_tol ? RISM3D_OK : RISM3D_NONCONVERGED;
+          // no polish: the FP32 phase's own end decides (mixed_tol or its
+          // floor reached); a polish must reach mixed_tol in FP64
+          const bool ok = cf.polish == 0 ? s.reached_mixed != 0 : rinf <= cf.mixed_tol;
+          s.status = ok ? RISM3D_OK : RISM3D_NONCONVERGED;
           s.active = 0;
           continue;
         }
@@ -525,14 +688,94 @@ void iterate_batch(Problem& pb, const Config& cf) {
         s.coef.assign(1, 1.0);
       }
     }
-    if (any_handover) convert_iterates(pb, handover, true);
+    if (any_rebase) {
+      std::vector<double> rinf, r2;
+      rebase(pb, cf, rebase_mask, true, rinf, r2);
+      after_rebase(rebase_mask, rinf, r2, false);
+    }
   }
-  // solutes that stopped inside the FP32 phase (NaN, limits): FP64 copy
-  std::vector<int> mask(B, 0);
-  bool any32 = false;
+  // solutes that stopped inside the FP32 phase (NaN): t = t_b + delta
   for (int b = 0; b < B; ++b)
-    if (pb.st[b].phase32) { mask[b] = 1; any32 = true; }
-  if (any32) convert_iterates(pb, mask, true);
+    if (pb.st[b].phase32) {
+      const int ch = (int)std::min<int64_t>(2048, (mP + NT - 1) / NT);
+      hipLaunchKernelGGL(add_delta, dim3(ch), dim3(NT), 0, pb.s, pb.st[b].t64.p, pb.st[b].dcur, mP);
+      pb.st[b].phase32 = 0;
+    }
+  HIPCHK(hipStreamSynchronize(pb.s));
+}
+
+// Closure continuation for solutes whose iteration diverged (max|F| not
+// finite): HNC and PSE-n (n >= 2) grow like exp(x) or x^n where the solute
+// field is strongly attractive, and from t = 0 the first steps can
+// overshoot into overflow.  Such solutes restart from t = 0 along
+// KH = PSE-1, PSE-2, ..., each rung converged loosely (max|F| <= 1e-2) and
+// handed to the next, the target closure last, to the full tolerance.
+// For HNC the ladder ends with PSE-4.  The fixed point is the target
+// closure's: only the starting point of its iteration changes.
+// Iteration counts add up over the rungs.
+void continuation(Problem& pb, const Config& cf, int target) {
+  std::vector<std::pair<int, int>> rungs;  // (closure enum, PSE order)
+  const int top = target == RISM3D_CLOSURE_HNC ? 5 : target - RISM3D_CLOSURE_PSE0;
+  if (target != RISM3D_CLOSURE_HNC && target < RISM3D_CLOSURE_PSE2) return;
+  rungs.push_back({RISM3D_CLOSURE_KH, 1});
+  for (int n = 2; n < top; ++n) rungs.push_back({RISM3D_CLOSURE_PSE0 + n, n});
+  rungs.push_back({target, target == RISM3D_CLOSURE_HNC ? 0 : top});
+  const int B = pb.B;
+  std::vector<int> redo(B, 0), tot(B, 0), tot32(B, 0);
+  bool any = false;
+  for (int b = 0; b < B; ++b) {
+    auto& s = pb.st[b];
+    s.active = 0;
+    if (s.status != RISM3D_NAN) continue;
+    redo[b] = 1;
+    any = true;
+    tot[b] = s.iters;
+    tot32[b] = s.iters32;
+    HIPCHK(hipMemsetAsync(s.t64.p, 0, pb.mP * 8, pb.s));
+  }
+  if (!any) return;
+  const std::vector<int> joined = redo;
+  const GridArgs g0 = pb.g;
+  for (size_t k = 0; k < rungs.size(); ++k) {
+    const bool last = k + 1 == rungs.size();
+    pb.g.closure = rungs[k].first;
+    pb.g.pse_order = rungs[k].second;
+    for (int b = 0; b < B; ++b) {
+      auto& s = pb.st[b];
+      if (!redo[b]) continue;
+      s.active = 1;
+      s.status = RISM3D_NONCONVERGED;
+      s.iters = s.iters32 = 0;
+      s.reached_mixed = 0;
+      s.best32 = INFINITY;
+      s.stall32 = 0;
+      s.polish = 0;
+      s.resid = s.best = INFINITY;
+      s.since_best = 0;
+    }
+    Config c = cf;
+    if (!last) {
+      c.tol = std::max(cf.tol, 1e-2);
+      c.mixed_tol = std::max(cf.mixed_tol, 1e-2);
+      c.polish = 0;
+    }
+    if (cf.verbose)
+      std::fprintf(stderr, "rism3d: continuation rung %d, closure %d\n", (int)k, rungs[k].first);
+    iterate_batch(pb, c);
+    for (int b = 0; b < B; ++b) {
+      if (!redo[b]) continue;
+      auto& s = pb.st[b];
+      tot[b] += s.iters;
+      tot32[b] += s.iters32;
+      if (s.status == RISM3D_NAN) redo[b] = 0;  // diverged on a rung: stays NAN
+    }
+  }
+  pb.g = g0;
+  for (int b = 0; b < B; ++b)
+    if (joined[b]) {
+      pb.st[b].iters = tot[b];
+      pb.st[b].iters32 = tot32[b];
+    }
 }
 
 struct Inputs {
@@ -859,7 +1102,12 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     s.t64 = DevBuf<double>(pb.mP);
     if (pb.mixed) s.t32 = DevBuf<float>(pb.mP);
     s.bv64 = DevBuf<double>(pb.mP);
-    if (pb.mixed) s.bv32 = DevBuf<float>(pb.mP);
+    if (pb.mixed) {
+      s.bv32 = DevBuf<float>(pb.mP);
+      s.r64 = DevBuf<double>(pb.mP);
+      s.fb32 = DevBuf<float>(pb.mP);
+      s.t32b = DevBuf<float>(pb.mP);
+    }
     s.W = DevBuf<cpx<double>>(mK);
     s.fs64 = DevBuf<cpx<double>>(g.K);
     if (pb.mixed) s.fs32 = DevBuf<cpx<float>>(g.K);
@@ -964,6 +1212,7 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   cf.tol = pd.norm_tol;
   cf.mixed_tol = in.opt.mixed_tol;
   iterate_batch(pb, cf);
+  if (in.opt.continuation) continuation(pb, cf, pd.closure);
 
   // observables in FP64 from the converged t (hnc3d.c:2157-2268)
   {
@@ -1057,6 +1306,7 @@ void rism3d_options_default(rism3d_options* opt) {
   opt->verbose = 0;
   opt->device = 0;
   opt->dielectric = 0.0;
+  opt->continuation = 1;
 }
 
 int rism3d_solvent_chi_internal(const rism3d_problem* pd, int m, const rism3d_site* solvent,

@@ -63,7 +63,8 @@ class Options(ctypes.Structure):
                 ("precision", ctypes.c_int), ("polish_iters", ctypes.c_int),
                 ("mixed_tol", ctypes.c_double), ("diis_depth", ctypes.c_int),
                 ("diis_step", ctypes.c_double), ("verbose", ctypes.c_int),
-                ("device", ctypes.c_int), ("dielectric", ctypes.c_double)]
+                ("device", ctypes.c_int), ("dielectric", ctypes.c_double),
+                ("continuation", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -192,7 +193,8 @@ def make_options(settings=None, **kw):
     for key, field in (("precision", "precision"), ("polish-iters", "polish_iters"),
                        ("mixed-tol", "mixed_tol"), ("diis-depth", "diis_depth"),
                        ("diis-step", "diis_step"), ("verbose", "verbose"),
-                       ("device", "device"), ("dielectric", "dielectric")):
+                       ("device", "device"), ("dielectric", "dielectric"),
+                       ("continuation", "continuation")):
         if key in s:
             v = s[key]
             if key == "precision" and isinstance(v, str):
@@ -275,6 +277,7 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
         out["free-energy-response"] = resp
     if dsite is not None:
         out["free-energy-dsite"] = dsite
+    if dsite is not None or resp is not None:
         out["adjoint-iterations"] = res.adjoint_iterations
     if tok is not None:
         out["RESTART"] = tok.value

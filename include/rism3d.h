@@ -98,7 +98,9 @@ typedef struct rism3d_options {
   int precision;       /* enum rism3d_precision                            */
   int polish_iters;    /* MIXED: FP64 steps after the FP32 phase, towards
                           norm_tol; NONCONVERGED if the last one is above
-                          mixed_tol                                        */
+                          mixed_tol.  0: OK when the FP32 phase reached
+                          mixed_tol or its floor (residual then reports
+                          the FP64 residual of the FP32 iterate)          */
   double mixed_tol;    /* MIXED: max-norm residual at which FP32 hands over
                           (or its FP32 floor, detected as a stall below
                           1e-3)                                            */
@@ -109,6 +111,12 @@ typedef struct rism3d_options {
   double dielectric;   /* chi_k == NULL only: DRISM target epsilon of the
                           internal 1D chi producer ("dielectric" setting,
                           rism.f90:346); 0 selects plain RISM              */
+  int continuation;    /* HNC, PSE-n >= 2: a solute whose iteration
+                          diverges (NAN) restarts from t = 0 along KH,
+                          PSE-2, ... to the target closure, each rung
+                          loosely converged (default 1; 0: report NAN).
+                          The reference has no such fallback; the fixed
+                          point is unchanged                               */
 } rism3d_options;
 
 typedef struct rism3d_result {

@@ -299,14 +299,41 @@ class Solute3D:
         out = irfft(pd, self.star(cf)) / pd.L3
         return out if flag else out - dt
 
-    def solve(self, t0=None, atol=1e-11, verbose=False):
+    def ladder(self):
+        """Closures tried before the target when its iteration diverges:
+        KH (PSE-1), PSE-2, ... (PSE-4 last before HNC).  Not in the
+        reference; only the starting point of the target's iteration
+        changes, not its root."""
+        cl = self.pd.closure
+        if cl == C.HNC:
+            top = 5
+        elif C.PSE0 + 2 <= cl <= C.PSE7:
+            top = cl - C.PSE0
+        else:
+            return []
+        return [C.KH] + [C.PSE0 + n for n in range(2, top)]
+
+    def solve(self, t0=None, atol=1e-11, verbose=False, continuation=True):
         """MDIIS to atol; if it stalls, Newton-GMRES from its best iterate.
-        Only the root matters for parity (see nksolve)."""
+        If it diverges (non-finite residual), the closure ladder from t = 0,
+        each rung to max|F| <= 1e-2.  Only the root matters for parity (see
+        nksolve)."""
         t0 = np.zeros_like(self.v) if t0 is None else t0
         try:
             t, n_pre = mdiis(self.iterate, t0, atol=atol, max_iter=1500)
             return t, {"n_pre": n_pre, "iters": 0, "n_f": n_pre, "n_j": 0}
-        except NotConverged:
+        except NotConverged as e:
+            rungs = self.ladder() if continuation else []
+            if rungs and "not finite" in str(e):
+                target = self.pd.closure
+                t0 = np.zeros_like(self.v)
+                try:
+                    for cl in rungs:
+                        self.pd.closure = cl
+                        t0, _ = mdiis(self.iterate, t0, atol=1e-2, max_iter=1500)
+                finally:
+                    self.pd.closure = target
+                return self.solve(t0=t0, atol=atol, verbose=verbose, continuation=False)
             t0, n_pre = mdiis(self.iterate, t0, atol=1e-4)
         jac = self.jacobian if self.pd.closure != C.PY else None
         t, stats = newton_krylov(self.iterate, t0, J=jac, atol=atol,

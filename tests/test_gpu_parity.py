@@ -161,7 +161,7 @@ def test_mixed_vs_fp64(chi_w):
         dmu = abs(m["free-energy"] - f64["free-energy"])
         dg = np.max(np.abs(m["GUV"] - f64["GUV"]))
         print(f"  polish {polish}: |dmu| {dmu:.2e} max|dg| {dg:.2e} iters {m['iterations']} "
-              f"fp32 {m['iterations-fp32']}", flush=True)
+              f"fp32 {m['iterations-fp32']} residual {m['residual']:.2e} {m['status']}", flush=True)
         assert m["status"] == "OK"
         assert dmu < 0.01
 
@@ -252,3 +252,29 @@ def test_forces_scm_anchors_gpu():
         out = R.hnc3d_run_solute(name, "OW", s, chi=None)
         print(f"  {name}: mu {out['free-energy']:.12f} reference {ref}", flush=True)
         assert abs(out["free-energy"] - ref) < 1e-8
+
+
+def test_continuation_hnc(chi_w):
+    """A solute whose HNC iteration diverges from t = 0 (synthetic seed 4,
+    6 sites): the closure ladder KH, PSE-2..4, HNC reaches the oracle's
+    root (which the oracle also reaches without a ladder, at MDIIS step
+    0.2, tests/test_oracle_continuation.py)."""
+    print("case continuation hnc seed4 n6 n32", flush=True)
+    from bgy3d_amd import synthetic as S
+    R = _R()
+    sol = list(S.solute(4, n=6))
+    s = _settings(32, L=8.0, **{"norm-tol": 1e-11, "max-iter": 3000})
+    out = R.hnc3d_run_solute(sol, W, s, chi=chi_w)
+    ref = O.Solute3D(_oracle_problem(s), _sites(W), sol, chi_w)
+    t, _ = ref.solve(atol=1e-11)
+    obs = ref.observables(t)
+    dmu = abs(out["free-energy"] - obs["mu"])
+    dg = np.max(np.abs(out["GUV"] - obs["g"]))
+    print(f"  status {out['status']} iters {out['iterations']} |dmu| {dmu:.2e} max|dg| {dg:.2e}",
+          flush=True)
+    assert out["status"] == "OK"
+    assert dmu < 1e-8 and dg < 1e-8
+    m = R.hnc3d_run_solute(sol, W, dict(s, precision="mixed"), chi=chi_w)
+    dmu = abs(m["free-energy"] - obs["mu"])
+    print(f"  mixed: status {m['status']} iters {m['iterations']} |dmu| {dmu:.2e}", flush=True)
+    assert m["status"] == "OK" and dmu < 0.01

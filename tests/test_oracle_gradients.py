@@ -63,7 +63,8 @@ def _solved(pd, sites, chi, solvent=W):
 def test_response_uncharged_equals_exact_gradient():
     """For an uncharged solute the reference's linear-response gradient
     (hnc3d.c:1969-2134) leaves nothing out: it equals the adjoint
-    gradient's x, y, z columns and central differences of mu."""
+    gradient's x, y, z columns, central differences of mu, and the TPT
+    gradient (hnc3d.c:1666-1742), exact here as the HNC mu is stationary."""
     chi = load_chi(CHI_W)
     pd = O.Problem.from_cli(8.0, 32, 1.6889, 0.0333295, rmax=40.0, nrad=1536)
     sites = [("A", (0.0, 0.0, 0.9), 3.16, 0.155, 0.0), ("B", (0.3, 0.1, -0.9), 3.0, 0.1, 0.0)]
@@ -71,6 +72,9 @@ def test_response_uncharged_equals_exact_gradient():
     resp = Gr.response_gradient(s, t)
     grad, _ = Gr.adjoint_gradient(s, t)
     np.testing.assert_allclose(resp, grad[:, :3], rtol=0, atol=1e-6 * np.max(np.abs(resp)))
+    # and so is the TPT form: uncharged, the mu functional is stationary
+    tpt = Gr.tpt_gradient(pd, W, sites, s.observables(t)["g"])
+    np.testing.assert_allclose(tpt, grad[:, :3], rtol=0, atol=1e-6 * np.max(np.abs(resp)))
     h = 1e-4
     for a, d in ((0, 2), (1, 0)):
         mus = []
