@@ -51,6 +51,22 @@ def chi_solvent():
         return np.load(path)
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE passes over the same
+    command, each scaled by its reading for a plain copy of known size), or
+    None when no summary for this workload exists."""
+    path = os.path.join(ROOT, "profiles", "traffic_batch64_128.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("workload") != workload:
+        return None
+    return rec.get("bytes_per_launch", {}).get(kernel)
+
+
 def cpu_baseline(solute, L_full, chi, fp64_iters, n_eval=4):
     """The oracle (numpy FP64, 1 core) on a bounded sample: setup of one
     solute plus n_eval residual evaluations at 128^3, scaled by the mean
@@ -86,9 +102,13 @@ def main():
     ap.add_argument("--diis-depth", type=int, default=8)
     ap.add_argument("--diis-step", type=float, default=0.5)
     ap.add_argument("--polish", type=int, default=3)
+    ap.add_argument("--skip-fp64", action="store_true",
+                    help="time the mixed batch only (profiling runs: the FP64 comparison "
+                         "batch and its fields are left out)")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first run a plain 512 MiB device-to-device copy (3 launches), the "
-                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
+                    help="first run a plain elementwise copy of 512 MiB (b = 1.0 * a, FP64, "
+                         "3 launches), the reference kernel for scaling FETCH_SIZE / "
+                         "WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -105,7 +125,7 @@ def main():
         a = torch.ones(n, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         for _ in range(3):
-            b.copy_(a)
+            torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
         del a, b
 
@@ -125,21 +145,13 @@ def main():
     t_mixed = (time.perf_counter() - t0) / args.steps
     prof = R.profile_read()
 
-    t0 = time.perf_counter()
-    res_fp64 = R.batch_solve(solutes, W, fp64, chi)
-    t_fp64 = time.perf_counter() - t0
-
-    ok = [a["status"] == "OK" and b["status"] == "OK" for a, b in zip(res_mixed, res_fp64)]
-    dmu = [abs(a["free-energy"] - b["free-energy"]) for a, b in zip(res_mixed, res_fp64)]
     n = len(solutes)
     value = n / t_mixed
-    fp64_rate = n / t_fp64
-
-    worst = int(np.argmax(dmu))
     dom = max(prof, key=lambda k: prof[k][0])
     ms, by, nl = prof[dom]
     achieved = by / (ms * 1e-3) / 1e9
-    fp64_iters = float(np.mean([b["iterations"] for b in res_fp64]))
+    workload = (f"{n} synthetic solutes (seeds 0-{n - 1}, 5-30 sites, Q in 0/+-1) in PR-SPC/E "
+                f"water, KH + tau renorm, 128^3 grid, box {L_full} A, one batched solve per step")
     line = {
         "metric": METRIC,
         "value": value,
@@ -153,34 +165,48 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 bulk + f64 state/residual/mu",
         "data": "synthetic",
-        "config": {"workload": f"{n} synthetic solutes (seeds 0-{n - 1}, 5-30 sites, Q in 0/+-1) "
-                               f"in PR-SPC/E water, KH + tau renorm, 128^3 grid, box {L_full} A, "
-                               "one batched solve per step",
-                   "grid": 128, "box_A": L_full, "solutes": n, "closure": "KH",
-                   "mixed_tol": 1e-5, "polish_iters": args.polish, "fp64_tol": 1e-10},
-        "fp64_solves_per_s": fp64_rate,
-        "mixed_over_fp64": value / fp64_rate,
-        "max_abs_dmu_kcal": max(dmu),
-        "all_converged": all(ok),
+        "config": {"workload": workload, "grid": 128, "box_A": L_full, "solutes": n,
+                   "closure": "KH", "mixed_tol": 1e-5, "polish_iters": args.polish,
+                   "fp64_tol": 1e-10},
+    }
+    fp64_iters = None
+    if not args.skip_fp64:
+        t0 = time.perf_counter()
+        res_fp64 = R.batch_solve(solutes, W, fp64, chi)
+        t_fp64 = time.perf_counter() - t0
+        ok = [a["status"] == "OK" and b["status"] == "OK" for a, b in zip(res_mixed, res_fp64)]
+        dmu = [abs(a["free-energy"] - b["free-energy"]) for a, b in zip(res_mixed, res_fp64)]
+        worst = int(np.argmax(dmu))
+        fp64_rate = n / t_fp64
+        fp64_iters = float(np.mean([b["iterations"] for b in res_fp64]))
+        line.update({
+            "fp64_solves_per_s": fp64_rate,
+            "mixed_over_fp64": value / fp64_rate,
+            "max_abs_dmu_kcal": max(dmu),
+            "all_converged": all(ok),
+            "mean_iters_fp64": fp64_iters,
+            "worst_solute": {"seed": worst, "dmu": dmu[worst],
+                             "mixed": {k: res_mixed[worst][k] for k in (
+                                 "free-energy", "iterations", "iterations-fp32", "residual",
+                                 "status")},
+                             "fp64": {k: res_fp64[worst][k] for k in (
+                                 "free-energy", "iterations", "residual", "status")}},
+        })
+    line.update({
         "mean_iters_mixed": float(np.mean([a["iterations"] for a in res_mixed])),
         "mean_iters_fp32": float(np.mean([a["iterations-fp32"] for a in res_mixed])),
-        "mean_iters_fp64": fp64_iters,
-        "worst_solute": {"seed": worst, "dmu": dmu[worst],
-                         "mixed": {k: res_mixed[worst][k] for k in ("free-energy", "iterations",
-                                                                    "iterations-fp32", "residual",
-                                                                    "status")},
-                         "fp64": {k: res_fp64[worst][k] for k in ("free-energy", "iterations",
-                                                                  "residual", "status")}},
         "diis": {"depth": args.diis_depth, "step": args.diis_step, "polish": args.polish},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                     "avg_launch_us": ms / nl * 1e3, "bytes_per_launch": by / nl},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload, dom),
+                     "kernel": dom, "avg_launch_us": ms / nl * 1e3, "bytes_per_launch": by / nl},
         "kernels": {k: {"avg_us": v[0] / max(v[2], 1) * 1e3,
                         "GB_s": (v[1] / (v[0] * 1e-3) / 1e9) if v[0] > 0 else None,
+                        "bytes_per_launch": v[1] / max(v[2], 1),
                         "launches": v[2]} for k, v in prof.items()},
-    }
+    })
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(solutes[0], L_full, chi, fp64_iters)
+        iters = fp64_iters if fp64_iters is not None else line["mean_iters_mixed"]
+        line["cpu_baseline"] = cpu_baseline(solutes[0], L_full, chi, iters)
     print(json.dumps(line), flush=True)
 
 

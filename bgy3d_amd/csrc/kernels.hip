@@ -43,6 +43,21 @@ __host__ __device__ constexpr int rows_for(int M) {
   return M <= 16 ? 64 : M <= 32 ? 32 : M <= 64 ? 16 : M <= 128 ? 8 : 4;
 }
 
+// 16-byte vectors of the working type: the streaming loops of the x-axis
+// kernels move 16 B per lane per array (global_load/store_dwordx4), four
+// times fewer memory instructions than scalar accesses.  Every array is
+// addressed at multiples of N0 >= 32 elements from a 256-byte aligned base.
+template <typename T> struct alignas(16) Vec16 {
+  static constexpr int W = 16 / sizeof(T);
+  T v[W];
+};
+template <typename T> __device__ __forceinline__ Vec16<T> ld16(const T* p) {
+  return *reinterpret_cast<const Vec16<T>*>(p);
+}
+template <typename T> __device__ __forceinline__ void st16(T* p, const Vec16<T>& x) {
+  *reinterpret_cast<Vec16<T>*>(p) = x;
+}
+
 // ------------------------------------------- 1. closure fused with x r2c
 // t = sum_j coef_j u_j (the MDIIS step; written back as the iterate) or the
 // given t, then c = closure(beta v, t), then the x-axis r2c of c.
@@ -65,25 +80,51 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
   const int nc = d.ncomb;
   T* ldsr = reinterpret_cast<T*>(lds);
   const int nvalid = min(ROWS, nrows - row0) * N0;
-  for (int e = threadIdx.x; e < ROWS * N0; e += NT) {
-    T c = T(0);
-    if (e < nvalid) {
-      T tv;
+  constexpr int VW = Vec16<T>::W;
+  for (int e = threadIdx.x * VW; e < ROWS * N0; e += NT * VW) {
+    Vec16<T> c;
+#pragma unroll
+    for (int k = 0; k < VW; ++k) c.v[k] = T(0);
+    if (e < nvalid) {  // nvalid is a multiple of N0: whole vectors
+      Vec16<T> tv;
       if (nc > 0) {
-        double s = 0.0;
-        for (int j = 0; j < nc; ++j) s += d.coef[j] * (double)static_cast<const T*>(d.u[j])[off + e];
-        if (d.shift_w != 0.0) s -= d.shift_w * (double)static_cast<const T*>(d.shift)[off + e];
-        tv = T(s);
-        t[e] = tv;
+        // all history loads first, then the sums: independent loads in flight
+        Vec16<T> u[MAX_DIIS];
+#pragma unroll
+        for (int j = 0; j < MAX_DIIS; ++j)
+          if (j < nc) u[j] = ld16(static_cast<const T*>(d.u[j]) + off + e);
+        double acc[VW];
+#pragma unroll
+        for (int k = 0; k < VW; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int j = 0; j < MAX_DIIS; ++j)
+          if (j < nc) {
+            const double cj = d.coef[j];
+#pragma unroll
+            for (int k = 0; k < VW; ++k) acc[k] += cj * (double)u[j].v[k];
+          }
+        if (d.shift_w != 0.0) {
+          const Vec16<T> sh = ld16(static_cast<const T*>(d.shift) + off + e);
+#pragma unroll
+          for (int k = 0; k < VW; ++k) acc[k] -= d.shift_w * (double)sh.v[k];
+        }
+#pragma unroll
+        for (int k = 0; k < VW; ++k) tv.v[k] = T(acc[k]);
+        st16(t + e, tv);
       } else {
-        tv = t[e];
+        tv = ld16(t + e);
       }
-      if (d.mode == 0) c = closure_c<T>(g.closure, g.pse_order, bv[e], tv);
-      else if (d.mode == 5)  // bv holds x_b (PY: exp(-beta v) - 1, c linear in t)
-        c = (g.closure == 2) ? bv[e] * tv : closure_dc<T>(g.closure, g.pse_order, bv[e], tv);
-      else c = tv;
+      const Vec16<T> b = ld16(bv + e);
+#pragma unroll
+      for (int k = 0; k < VW; ++k) {
+        if (d.mode == 0) c.v[k] = closure_c<T>(g.closure, g.pse_order, b.v[k], tv.v[k]);
+        else if (d.mode == 5)  // bv holds x_b (PY: exp(-beta v) - 1, c linear in t)
+          c.v[k] = (g.closure == 2) ? b.v[k] * tv.v[k]
+                                    : closure_dc<T>(g.closure, g.pse_order, b.v[k], tv.v[k]);
+        else c.v[k] = tv.v[k];
+      }
     }
-    ldsr[e] = c;  // z_n = c_{2n} + i c_{2n+1}, row stride M complex
+    st16(ldsr + e, c);  // z_n = c_{2n} + i c_{2n+1}, row stride M complex
   }
   __syncthreads();
   fft_lines<T, M, ROWS, NT, false>(lds, M, 1, tw_half);
@@ -115,13 +156,13 @@ fft_y(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   cpx<T>* W = static_cast<cpx<T>*>(d.W) + site * g.K + (int64_t)z * g.N1 * g.NH + kx0;
   for (int e = threadIdx.x; e < N1 * LINES; e += NT) {
     const int y = e / LINES, l = e - y * LINES;
-    lds[l * LS + y] = (kx0 + l < g.NH) ? W[y * g.NH + l] : cpx<T>{T(0), T(0)};
+    lds[l * LS + y] = (kx0 + l < g.NX) ? W[y * g.NH + l] : cpx<T>{T(0), T(0)};
   }
   __syncthreads();
   fft_lines<T, N1, LINES, NT, INV>(lds, LS, 1, tw);
   for (int e = threadIdx.x; e < N1 * LINES; e += NT) {
     const int y = e / LINES, l = e - y * LINES;
-    if (kx0 + l < g.NH) W[y * g.NH + l] = lds[l * LS + y];
+    if (kx0 + l < g.NX) W[y * g.NH + l] = lds[l * LS + y];
   }
 }
 
@@ -172,6 +213,7 @@ ktab3(GridArgs g, const double* __restrict__ chi_tab, const double* __restrict__
   const int64_t stride = (int64_t)gridDim.x * NT;
   for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.K; e += stride) {
     const int x = (int)(e % g.NH);
+    if (x >= g.NX) continue;  // row padding
     const int y = (int)((e / g.NH) % g.N1);
     const int z = (int)(e / ((int64_t)g.NH * g.N1));
     const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
@@ -220,7 +262,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     cpx<T>* L = lds + s * LINES * LS;
     for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
       const int z = e / LINES, l = e - z * LINES;
-      L[l * LS + z] = (kx0 + l < g.NH) ? Ws[z * zs + l] : cpx<T>{T(0), T(0)};
+      L[l * LS + z] = (kx0 + l < g.NX) ? Ws[z * zs + l] : cpx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
@@ -231,7 +273,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
       const cpx<T>* L = lds + s * LINES * LS;
       for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
         const int z = e / LINES, l = e - z * LINES;
-        if (kx0 + l < g.NH) Ws[z * zs + l] = L[l * LS + z];
+        if (kx0 + l < g.NX) Ws[z * zs + l] = L[l * LS + z];
       }
     }
     return;
@@ -243,7 +285,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   const double ky = iy * g.dky;
   for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
     const int z = e / LINES, l = e - z * LINES;
-    if (kx0 + l >= g.NH) continue;
+    if (kx0 + l >= g.NX) continue;
     const int64_t kk = base + z * zs + l;   // k-grid index of this point
     const cpx<T> fs = Fs[z * zs + l];
     cpx<T> c[MAX_SITES];
@@ -290,7 +332,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     const cpx<T>* L = lds + s * LINES * LS;
     for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
       const int z = e / LINES, l = e - z * LINES;
-      if (kx0 + l < g.NH) Ws[z * zs + l] = L[l * LS + z];
+      if (kx0 + l < g.NX) Ws[z * zs + l] = L[l * LS + z];
     }
   }
 }
@@ -356,21 +398,41 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
 #pragma unroll
   for (int j = 0; j < MAX_DIIS; ++j) dot[j] = 0.0;
   const int mode = d.mode;
-  for (int e = threadIdx.x; e < vrows * N0; e += NT) {
-    const T tv = t[e];
-    T rv;
-    if (mode == 0) rv = zr[e] - tv;                                    // F(t)
-    else if (mode == 5) rv = static_cast<const T*>(d.fb)[off + e] + zr[e] - tv;  // defect
-    else if (mode == 1) rv = T(d.dt[off + e]) * zr[e] + T(d.gv[off + e]) - tv;  // adjoint
-    else rv = zr[e];                                                   // K t
-    r[e] = rv;
-    if (un) un[e] = tv + lam * rv;
-    const double rd = (double)rv;
-    mx = fmax(mx, fabs(rd));
-    ss += rd * rd;
+  constexpr int VW = Vec16<T>::W;
+  for (int e = threadIdx.x * VW; e < vrows * N0; e += NT * VW) {
+    const Vec16<T> tv = ld16(t + e);
+    const Vec16<T> z = *reinterpret_cast<const Vec16<T>*>(zr + e);
+    Vec16<T> rh[MAX_DIIS];
 #pragma unroll
     for (int j = 0; j < MAX_DIIS; ++j)
-      if (j < nd) dot[j] += rd * (double)static_cast<const T*>(d.rh[j])[off + e];
+      if (j < nd) rh[j] = ld16(static_cast<const T*>(d.rh[j]) + off + e);
+    Vec16<T> rv, uv;
+    if (mode == 5) {  // defect correction: F(t_b) + K dc - delta
+      const Vec16<T> f = ld16(static_cast<const T*>(d.fb) + off + e);
+#pragma unroll
+      for (int k = 0; k < VW; ++k) rv.v[k] = f.v[k] + z.v[k] - tv.v[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < VW; ++k) {
+        if (mode == 0) rv.v[k] = z.v[k] - tv.v[k];                                  // F(t)
+        else if (mode == 1)                                                          // adjoint
+          rv.v[k] = T(d.dt[off + e + k]) * z.v[k] + T(d.gv[off + e + k]) - tv.v[k];
+        else rv.v[k] = z.v[k];                                                       // K t
+      }
+    }
+    st16(r + e, rv);
+#pragma unroll
+    for (int k = 0; k < VW; ++k) uv.v[k] = tv.v[k] + lam * rv.v[k];
+    if (un) st16(un + e, uv);
+#pragma unroll
+    for (int k = 0; k < VW; ++k) {
+      const double rd = (double)rv.v[k];
+      mx = fmax(mx, fabs(rd));
+      ss += rd * rd;
+#pragma unroll
+      for (int j = 0; j < MAX_DIIS; ++j)
+        if (j < nd) dot[j] += rd * (double)rh[j].v[k];
+    }
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   mx = warp_max(mx);
@@ -566,6 +628,7 @@ form_factor(const SitesDesc* __restrict__ sites, GridArgs g, cpx<double>* const*
   const int64_t stride = (int64_t)gridDim.x * NT;
   for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.K; e += stride) {
     const int x = (int)(e % g.NH);
+    if (x >= g.NX) continue;  // row padding
     const int y = (int)((e / g.NH) % g.N1);
     const int z = (int)(e / ((int64_t)g.NH * g.N1));
     const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
@@ -846,6 +909,7 @@ grad_tau(GridArgs g, SitesDesc sd, const cpx<double>* __restrict__ Lam,
   const int64_t stride = (int64_t)gridDim.x * NT;
   for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.K; e += stride) {
     const int x = (int)(e % g.NH);
+    if (x >= g.NX) continue;  // row padding
     const int y = (int)((e / g.NH) % g.N1);
     const int z = (int)(e / ((int64_t)g.NH * g.N1));
     const int iy = (y <= g.N1 / 2) ? y : y - g.N1;

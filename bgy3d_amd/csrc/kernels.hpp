@@ -46,10 +46,13 @@ constexpr int NPART = 2 + MAX_DIIS;   // doubles per residual workgroup
 
 // Grid and physics constants shared by a batch.
 struct GridArgs {
-  int N0, N1, N2, NH;  // NH = N0/2 + 1
+  int N0, N1, N2;
+  int NX;              // N0/2 + 1: kx points of the half-complex spectrum
+  int NH;              // row stride of spectral arrays: NX rounded up to 16,
+                       // so that kx tiles start on 128-byte lines
   int m;               // solvent sites
   int nrad;            // 1D table length
-  int64_t P, K;        // N0 N1 N2, NH N1 N2
+  int64_t P, K;        // N0 N1 N2, NH N1 N2 (K counts the padding)
   double dkx, dky, dkz;  // 2 pi / L
   double dk_tab;       // pi / rmax, 1D table spacing
   double a_scale;      // h^3 / L^3: forward h^3 and inverse 1/L^3 folded

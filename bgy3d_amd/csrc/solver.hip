@@ -102,11 +102,11 @@ int zstar_lines(int N2, int m) {
   return L;
 }
 template <typename T>
-void launch_zstar(int N2, int NH, int N1, int B, hipStream_t s, const SoluteDesc* sd,
+void launch_zstar(int N2, int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd,
                   const GridArgs& g, const cpx<T>* tw, const T* chi, const T* tau) {
   const int L = zstar_lines<T>(N2, g.m);
   const size_t lds = (size_t)g.m * L * (N2 + 1) * sizeof(cpx<T>);
-  const dim3 grid = g.zorder ? dim3((NH + L - 1) / L, N1, B) : dim3(B, (NH + L - 1) / L, N1);
+  const dim3 grid = g.zorder ? dim3((NX + L - 1) / L, N1, B) : dim3(B, (NX + L - 1) / L, N1);
   switch (L) {
     case 2: launch_zstar_l<T, 2>(N2, grid, lds, s, sd, g, tw, chi, tau); break;
     case 4: launch_zstar_l<T, 4>(N2, grid, lds, s, sd, g, tw, chi, tau); break;
@@ -307,26 +307,27 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
   if (fp32) {
     using T = float;
     constexpr int LN = y_lines<T>();
-    const dim3 gy((g.NH + LN - 1) / LN, g.m * g.N2, pb.B);
+    const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
     mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p); mark(0, 1);
     mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(1, 1);
-    mark(2, 0); launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
+    mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
     mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(3, 1);
     mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p); mark(4, 1);
   } else {
     using T = double;
     constexpr int LN = y_lines<T>();
-    const dim3 gy((g.NH + LN - 1) / LN, g.m * g.N2, pb.B);
+    const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
     mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p); mark(0, 1);
     mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(1, 1);
-    mark(2, 0); launch_zstar<T>(g.N2, g.NH, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p); mark(2, 1);
+    mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p); mark(2, 1);
     mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(3, 1);
     mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p); mark(4, 1);
   }
   hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, pb.d_sd.p, pb.nparts, pb.d_out.p);
   if (pf.on) {
     const double sc = fp32 ? 4.0 : 8.0;  // working-type scalar
-    const double mP = (double)g.m * g.P, mK = (double)g.m * g.K;
+    const double Kv = (double)g.NX * g.N1 * g.N2;  // spectral points without padding
+    const double mP = (double)g.m * g.P, mK = (double)g.m * Kv;
     double b0 = 0, b4 = 0;
     int na = 0;
     for (auto& d : sd) {
@@ -337,9 +338,9 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
       // W in, t in, r and u out, r_j in (+ F(t_b) in, defect correction)
       b4 += mK * 2 * sc + mP * sc + 2 * mP * sc + d.ndot * mP * sc + (d.mode == 5 ? mP * sc : 0);
     }
-    const double tables = (double)(g.m * (g.m + 1) / 2 + g.m) * g.K * sc;
+    const double tables = (double)(g.m * (g.m + 1) / 2 + g.m) * Kv * sc;
     const double per[5] = {b0, na * 2 * mK * 2 * sc,
-                           na * (2 * mK * 2 * sc + (double)g.K * 2 * sc) + tables,
+                           na * (2 * mK * 2 * sc + Kv * 2 * sc) + tables,
                            na * 2 * mK * 2 * sc, b4};
     HIPCHK(hipEventSynchronize(pf.ev[4][1]));
     for (int k = 0; k < 5; ++k) {
@@ -973,10 +974,10 @@ void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& 
       using T = double;
       constexpr int LN = y_lines<T>();
       const dim3 gx(resid_grid_x(g), g.m, B);
-      const dim3 gy((g.NH + LN - 1) / LN, g.m * g.N2, B);
+      const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, B);
       launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
       launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
-      launch_zstar<T>(g.N2, g.NH, g.N1, B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p);
+      launch_zstar<T>(g.N2, g.NX, g.N1, B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p);
     }
     hipLaunchKernelGGL(adj_coef, dim3(cp), dim3(NT), 0, pb.s, g, sites[b], tsave[b].p, s.bv64.p,
                        (const double*)s.R.p, pd.L[0], pd.L[1], pd.L[2], pb.h[0], pb.h[1],
@@ -1026,7 +1027,9 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   for (int d = 0; d < 3; ++d) { pb.N[d] = pd.N[d]; pb.L[d] = pd.L[d]; pb.h[d] = pd.L[d] / pd.N[d]; }
   GridArgs& g = pb.g;
   std::memset(&g, 0, sizeof g);
-  g.N0 = pd.N[0]; g.N1 = pd.N[1]; g.N2 = pd.N[2]; g.NH = pd.N[0] / 2 + 1;
+  g.N0 = pd.N[0]; g.N1 = pd.N[1]; g.N2 = pd.N[2];
+  g.NX = pd.N[0] / 2 + 1;
+  g.NH = (g.NX + 15) / 16 * 16;
   g.m = in.m;
   g.nrad = pd.nrad;
   g.P = (int64_t)g.N0 * g.N1 * g.N2;
