@@ -54,6 +54,13 @@ __host__ __device__ constexpr int stage_ns(int n, int s) {  // product of earlie
   return p;
 }
 
+// LDS position of point p of a line, and the line stride (one point of
+// padding, so consecutive lines start on different banks).  Padding inside
+// the lines (one point per 16) was measured slower on gfx950: more
+// SQ_LDS_BANK_CONFLICT cycles in the y and z passes, not fewer.
+__host__ __device__ constexpr int lpos(int p) { return p; }
+__host__ __device__ constexpr int lstride_for(int n) { return n + 1; }
+
 template <typename T, int R, bool INV>
 __device__ __forceinline__ void dft_small(cpx<T> (&v)[4]) {
   if constexpr (R == 2) {
@@ -79,11 +86,11 @@ __device__ __forceinline__ void dft_small(cpx<T> (&v)[4]) {
   }
 }
 
-// One Stockham stage over LINES lines of length N in LDS.
-// Element (line l, point p) lives at buf[l * lstride + p * pstride].
+// One Stockham stage over the first nl of LINES lines of length N in LDS.
+// Element (line l, point p) lives at buf[l * lstride + lpos(p)].
 // tw[j] = exp(-2 pi i j / N), j < N (global memory, L1/L2 resident).
 template <typename T, int N, int LINES, int NT, int STAGE, bool INV>
-__device__ __forceinline__ void stockham_stage(cpx<T>* buf, int lstride, int pstride,
+__device__ __forceinline__ void stockham_stage(cpx<T>* buf, int lstride, int nl,
                                                const cpx<T>* __restrict__ tw) {
   constexpr int R = radix_at(N, STAGE);
   constexpr int NS = stage_ns(N, STAGE);
@@ -96,13 +103,13 @@ __device__ __forceinline__ void stockham_stage(cpx<T>* buf, int lstride, int pst
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int b = tid + p * NT;
-    if (PER * NT == BF || b < BF) {
-      const int line = b / NB;
+    const int line = b / NB;
+    if ((PER * NT == BF || b < BF) && line < nl) {
       const int j = b - line * NB;
       const int k = j % NS;
       cpx<T>* lb = buf + line * lstride;
 #pragma unroll
-      for (int q = 0; q < R; ++q) v[p][q] = lb[(j + q * NB) * pstride];
+      for (int q = 0; q < R; ++q) v[p][q] = lb[lpos(j + q * NB)];
       if constexpr (NS > 1) {
 #pragma unroll
         for (int q = 1; q < R; ++q) {
@@ -117,25 +124,26 @@ __device__ __forceinline__ void stockham_stage(cpx<T>* buf, int lstride, int pst
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int b = tid + p * NT;
-    if (PER * NT == BF || b < BF) {
-      const int line = b / NB;
+    const int line = b / NB;
+    if ((PER * NT == BF || b < BF) && line < nl) {
       const int j = b - line * NB;
       const int k = j % NS;
       const int base = (j / NS) * NS * R + k;
       cpx<T>* lb = buf + line * lstride;
 #pragma unroll
-      for (int q = 0; q < R; ++q) lb[(base + q * NS) * pstride] = v[p][q];
+      for (int q = 0; q < R; ++q) lb[lpos(base + q * NS)] = v[p][q];
     }
   }
   __syncthreads();
 }
 
+// FFT of the first nl (<= LINES) lines, each at buf + l * lstride.
 template <typename T, int N, int LINES, int NT, bool INV, int S = 0>
-__device__ __forceinline__ void fft_lines(cpx<T>* buf, int lstride, int pstride,
+__device__ __forceinline__ void fft_lines(cpx<T>* buf, int lstride, int nl,
                                           const cpx<T>* __restrict__ tw) {
   if constexpr (S < num_stages(N)) {
-    stockham_stage<T, N, LINES, NT, S, INV>(buf, lstride, pstride, tw);
-    fft_lines<T, N, LINES, NT, INV, S + 1>(buf, lstride, pstride, tw);
+    stockham_stage<T, N, LINES, NT, S, INV>(buf, lstride, nl, tw);
+    fft_lines<T, N, LINES, NT, INV, S + 1>(buf, lstride, nl, tw);
   }
 }
 

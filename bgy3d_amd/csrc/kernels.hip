@@ -68,7 +68,8 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
               const cpx<T>* __restrict__ tw_half, const cpx<T>* __restrict__ tw_full) {
   constexpr int M = N0 / 2;
   constexpr int ROWS = rows_for(M);
-  __shared__ cpx<T> lds[ROWS * M];
+  constexpr int LSM = lstride_for(M);
+  __shared__ cpx<T> lds[ROWS * LSM];
   const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
   const int site = blockIdx.y;
@@ -78,8 +79,8 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
   T* __restrict__ t = static_cast<T*>(d.t) + off;
   const T* __restrict__ bv = static_cast<const T*>(d.bv) + off;
   const int nc = d.ncomb;
-  T* ldsr = reinterpret_cast<T*>(lds);
-  const int nvalid = min(ROWS, nrows - row0) * N0;
+  const int vrows = min(ROWS, nrows - row0);
+  const int nvalid = vrows * N0;
   constexpr int VW = Vec16<T>::W;
   for (int e = threadIdx.x * VW; e < ROWS * N0; e += NT * VW) {
     Vec16<T> c;
@@ -124,17 +125,21 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
         else c.v[k] = tv.v[k];
       }
     }
-    st16(ldsr + e, c);  // z_n = c_{2n} + i c_{2n+1}, row stride M complex
+    // z_n = c_{2n} + i c_{2n+1} of row e / N0
+    cpx<T>* zrow = lds + (e / N0) * LSM;
+    const int n0 = (e % N0) / 2;
+#pragma unroll
+    for (int h = 0; h < VW / 2; ++h) zrow[lpos(n0 + h)] = cpx<T>{c.v[2 * h], c.v[2 * h + 1]};
   }
   __syncthreads();
-  fft_lines<T, M, ROWS, NT, false>(lds, M, 1, tw_half);
+  fft_lines<T, M, ROWS, NT, false>(lds, LSM, vrows, tw_half);
   cpx<T>* W = static_cast<cpx<T>*>(d.W) + site * g.K + (int64_t)row0 * g.NH;
   const int nout = min(ROWS, nrows - row0) * (M + 1);
   for (int e = threadIdx.x; e < nout; e += NT) {
     const int row = e / (M + 1);
     const int k = e - row * (M + 1);
-    const cpx<T> zk = lds[row * M + (k == M ? 0 : k)];
-    const cpx<T> zm = cconj(lds[row * M + (k == 0 ? 0 : M - k)]);
+    const cpx<T> zk = lds[row * LSM + lpos(k == M ? 0 : k)];
+    const cpx<T> zm = cconj(lds[row * LSM + lpos(k == 0 ? 0 : M - k)]);
     const cpx<T> s = cadd(zk, zm);
     const cpx<T> dd = rot90<T, false>(cmul(tw_full[k], csub(zk, zm)));  // -i w (Zk - conj Zm)
     W[row * g.NH + k] = cpx<T>{T(0.5) * (s.re + dd.re), T(0.5) * (s.im + dd.im)};
@@ -146,7 +151,7 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
 template <typename T, int N1, int LINES, bool INV>
 __global__ void __launch_bounds__(NT)
 fft_y(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw) {
-  constexpr int LS = N1 + 1;
+  constexpr int LS = lstride_for(N1);
   __shared__ cpx<T> lds[LINES * LS];
   const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
@@ -156,13 +161,13 @@ fft_y(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   cpx<T>* W = static_cast<cpx<T>*>(d.W) + site * g.K + (int64_t)z * g.N1 * g.NH + kx0;
   for (int e = threadIdx.x; e < N1 * LINES; e += NT) {
     const int y = e / LINES, l = e - y * LINES;
-    lds[l * LS + y] = (kx0 + l < g.NX) ? W[y * g.NH + l] : cpx<T>{T(0), T(0)};
+    lds[l * LS + lpos(y)] = (kx0 + l < g.NX) ? W[y * g.NH + l] : cpx<T>{T(0), T(0)};
   }
   __syncthreads();
-  fft_lines<T, N1, LINES, NT, INV>(lds, LS, 1, tw);
+  fft_lines<T, N1, LINES, NT, INV>(lds, LS, min(LINES, g.NX - kx0), tw);
   for (int e = threadIdx.x; e < N1 * LINES; e += NT) {
     const int y = e / LINES, l = e - y * LINES;
-    if (kx0 + l < g.NX) W[y * g.NH + l] = lds[l * LS + y];
+    if (kx0 + l < g.NX) W[y * g.NH + l] = lds[l * LS + lpos(y)];
   }
 }
 
@@ -238,13 +243,13 @@ ktab3(GridArgs g, const double* __restrict__ chi_tab, const double* __restrict__
 // tile), y = ceil(NH / LINES), z = ky row; zorder 1: x = kx tile, y = ky
 // row, z = solute (concurrent workgroups stream contiguous planes of one
 // solute).  Dynamic LDS:
-// m * LINES * (N2 + 1) complex.  chi3: [m(m+1)/2][K] chi_ij - delta_ij;
+// m * LINES * lstride_for(N2) complex.  chi3: [m(m+1)/2][K] chi_ij - delta_ij;
 // tau3: [m][K] s_i(|k|), both on the k grid, in the working type.
 template <typename T, int N2, int LINES>
 __global__ void __launch_bounds__(NT)
 zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw,
       const T* __restrict__ chi3, const T* __restrict__ tau3) {
-  constexpr int LS = N2 + 1;
+  constexpr int LS = lstride_for(N2);
   extern __shared__ unsigned char smem[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
   // zorder 0: grid (solute, kx tile, ky); 1: grid (kx tile, ky, solute)
@@ -262,18 +267,19 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     cpx<T>* L = lds + s * LINES * LS;
     for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
       const int z = e / LINES, l = e - z * LINES;
-      L[l * LS + z] = (kx0 + l < g.NX) ? Ws[z * zs + l] : cpx<T>{T(0), T(0)};
+      L[l * LS + lpos(z)] = (kx0 + l < g.NX) ? Ws[z * zs + l] : cpx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
-  for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, false>(lds + s * LINES * LS, LS, 1, tw);
+  const int nl = min(LINES, g.NX - kx0);  // valid kx columns of this tile
+  for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, false>(lds + s * LINES * LS, LS, nl, tw);
   if (d.mode == 3) {  // forward spectrum only
     for (int s = 0; s < m; ++s) {
       cpx<T>* Ws = W + s * g.K;
       const cpx<T>* L = lds + s * LINES * LS;
       for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
         const int z = e / LINES, l = e - z * LINES;
-        if (kx0 + l < g.NX) Ws[z * zs + l] = L[l * LS + z];
+        if (kx0 + l < g.NX) Ws[z * zs + l] = L[l * LS + lpos(z)];
       }
     }
     return;
@@ -289,7 +295,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     const int64_t kk = base + z * zs + l;   // k-grid index of this point
     const cpx<T> fs = Fs[z * zs + l];
     cpx<T> c[MAX_SITES];
-    for (int s = 0; s < m; ++s) c[s] = lds[(s * LINES + l) * LS + z];
+    for (int s = 0; s < m; ++s) c[s] = lds[(s * LINES + l) * LS + lpos(z)];
     cpx<T> uc = {T(0), T(0)};
     if (!g.renorm && !linear) {
       // u_c(k) = 4 pi eps0^-1 exp(-k^2/4G^2)/k^2 F(k) s(k), 0 at k = 0
@@ -322,17 +328,17 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
       } else {
         out = cscale(csub(acc, cscale(uc, T(g.beta * g.q[i]))), T(g.inv_L3));
       }
-      lds[(i * LINES + l) * LS + z] = out;
+      lds[(i * LINES + l) * LS + lpos(z)] = out;
     }
   }
   __syncthreads();
-  for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, true>(lds + s * LINES * LS, LS, 1, tw);
+  for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, true>(lds + s * LINES * LS, LS, nl, tw);
   for (int s = 0; s < m; ++s) {
     cpx<T>* Ws = W + s * g.K;
     const cpx<T>* L = lds + s * LINES * LS;
     for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
       const int z = e / LINES, l = e - z * LINES;
-      if (kx0 + l < g.NX) Ws[z * zs + l] = L[l * LS + z];
+      if (kx0 + l < g.NX) Ws[z * zs + l] = L[l * LS + lpos(z)];
     }
   }
 }
@@ -358,7 +364,8 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   constexpr int M = N0 / 2;
   constexpr int ROWS = rows_for(M);
   __shared__ cpx<T> xb[ROWS * (M + 1)];
-  __shared__ cpx<T> zb[ROWS * M];
+  constexpr int LSM = lstride_for(M);
+  __shared__ cpx<T> zb[ROWS * LSM];
   __shared__ double red[NPART][NT / 64];
   const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
@@ -382,11 +389,10 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
     const cpx<T> a = cadd(xk, cconj(xm));
     const cpx<T> b = csub(xk, cconj(xm));
     const cpx<T> ib = rot90<T, true>(cmulc(b, tw_full[k]));  // i e^{+2 pi i k/N} b
-    zb[e] = cadd(a, ib);
+    zb[row * LSM + lpos(k)] = cadd(a, ib);
   }
   __syncthreads();
-  fft_lines<T, M, ROWS, NT, true>(zb, M, 1, tw_half);
-  const T* zr = reinterpret_cast<const T*>(zb);
+  fft_lines<T, M, ROWS, NT, true>(zb, LSM, vrows, tw_half);
   const int64_t off = site * g.P + (int64_t)row0 * N0;
   const T* __restrict__ t = static_cast<const T*>(d.t) + off;
   T* __restrict__ r = static_cast<T*>(d.r) + off;
@@ -401,7 +407,17 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   constexpr int VW = Vec16<T>::W;
   for (int e = threadIdx.x * VW; e < vrows * N0; e += NT * VW) {
     const Vec16<T> tv = ld16(t + e);
-    const Vec16<T> z = *reinterpret_cast<const Vec16<T>*>(zr + e);
+    Vec16<T> z;  // t' at points e.. of row e / N0: z_n = t'_{2n} + i t'_{2n+1}
+    {
+      const cpx<T>* zrow = zb + (e / N0) * LSM;
+      const int n0 = (e % N0) / 2;
+#pragma unroll
+      for (int h = 0; h < VW / 2; ++h) {
+        const cpx<T> zz = zrow[lpos(n0 + h)];
+        z.v[2 * h] = zz.re;
+        z.v[2 * h + 1] = zz.im;
+      }
+    }
     Vec16<T> rh[MAX_DIIS];
 #pragma unroll
     for (int j = 0; j < MAX_DIIS; ++j)
