@@ -494,7 +494,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
   for (int b = 0; b < B; ++b) {
     auto& s = pb.st[b];
     act[b] = s.active != 0;
-    s.phase32 = s.active && cf.precision == RISM3D_MIXED;
+    s.phase32 = s.active && cf.precision != RISM3D_FP64;
     s.gram.assign(D * D, 0.0);
     s.order.clear();
     s.coef.clear();
@@ -540,6 +540,14 @@ void iterate_batch(Problem& pb, const Config& cf) {
         continue;
       }
       if (rinf[b] <= cf.mixed_tol || rinf[b] <= cf.tol) { end_fp32(b, rinf[b], r2[b]); continue; }
+      if (cf.precision == RISM3D_FP32 && !seed) {
+        // FP32 only: no second base; the FP64 residual at the end is
+        // reported, OK if it reached mixed_tol
+        s.status = RISM3D_NONCONVERGED;
+        s.active = 0;
+        s.phase32 = 0;
+        continue;
+      }
       if (s.iters32 >= cf.max_iter) {
         s.status = RISM3D_NONCONVERGED;
         s.active = 0;
@@ -550,7 +558,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
       rebase_start(pb, cf, b, r2[b], seed);
     }
   };
-  if (cf.precision == RISM3D_MIXED) {
+  if (cf.precision != RISM3D_FP64) {
     std::vector<double> rinf, r2;
     rebase(pb, cf, act, false, rinf, r2);
     after_rebase(act, rinf, r2, true);
@@ -634,11 +642,14 @@ void iterate_batch(Problem& pb, const Config& cf) {
         // below mixed_tol / 2 (the FP64 test then decides), or stopped
         // falling (no 30% gain on its best in 12 passes, counted below
         // 1e-2 F(t_b)), or at the iteration limit.
+        // FP32 only: no rebase until the floor (stall below 1e-3) or
+        // mixed_tol / 2.
+        const bool only32 = cf.precision == RISM3D_FP32;
         if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; }
-        else if (rinf <= 1e-2 * s.rb_rinf) { s.stall32++; }
+        else if (rinf <= (only32 ? 1e-3 : 1e-2 * s.rb_rinf)) { s.stall32++; }
         // The new entry still enters the history (the rebase moves it).
-        if (rinf <= 1e-3 * s.rb_rinf || rinf <= 0.5 * cf.mixed_tol || s.stall32 >= 12 ||
-            s.iters32 >= cf.max_iter) {
+        if ((!only32 && rinf <= 1e-3 * s.rb_rinf) || rinf <= 0.5 * cf.mixed_tol ||
+            s.stall32 >= 12 || s.iters32 >= cf.max_iter) {
           rebase_mask[b] = 1;
           any_rebase = true;
         }
@@ -1027,7 +1038,7 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   Problem pb;
   pb.B = in.B;
   pb.D = in.opt.solver == RISM3D_SOLVER_PICARD ? 1 : in.opt.diis_depth;
-  pb.mixed = in.opt.precision == RISM3D_MIXED;
+  pb.mixed = in.opt.precision != RISM3D_FP64;
   for (int d = 0; d < 3; ++d) { pb.N[d] = pd.N[d]; pb.L[d] = pd.L[d]; pb.h[d] = pd.L[d] / pd.N[d]; }
   GridArgs& g = pb.g;
   std::memset(&g, 0, sizeof g);

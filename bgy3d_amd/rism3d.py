@@ -31,7 +31,7 @@ OK, NONCONVERGED, NAN, BAD_ARG, NO_DEVICE, OOM, UNSUPPORTED = range(7)
 STATUS_NAMES = ["OK", "NONCONVERGED", "NAN", "BAD_ARG", "NO_DEVICE", "OOM", "UNSUPPORTED"]
 
 SOLVER_MDIIS, SOLVER_PICARD = 0, 1
-FP64, MIXED = 0, 1
+FP64, MIXED, FP32 = 0, 1, 2
 
 
 class RismError(RuntimeError):
@@ -198,7 +198,7 @@ def make_options(settings=None, **kw):
         if key in s:
             v = s[key]
             if key == "precision" and isinstance(v, str):
-                v = {"fp64": FP64, "mixed": MIXED}[v]
+                v = {"fp64": FP64, "mixed": MIXED, "fp32": FP32}[v]
             setattr(opt, field, type(getattr(opt, field))(v))
     return opt
 

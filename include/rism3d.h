@@ -85,8 +85,13 @@ enum rism3d_solver {
 
 enum rism3d_precision {
   RISM3D_FP64 = 0,          /* every kernel in FP64                     */
-  RISM3D_MIXED = 1          /* FP32 FFT/closure/contraction, FP64 state,
-                               residual, DIIS and mu; FP64 polish        */
+  RISM3D_MIXED = 1,         /* FP32 passes on a correction around an FP64
+                               base (defect correction): FP64 residual at
+                               every rebase, FP64 polish; no FP32 floor  */
+  RISM3D_FP32 = 2           /* FP32 passes only, around the starting
+                               point: they stop at the FP32 floor (no
+                               rebase, no polish); the FP64 residual there
+                               is reported (precision studies)          */
 };
 
 /* The reference reads these from the Guile *settings* fluid

@@ -2,9 +2,11 @@
 
 Config 2 (precision study): one 120-site synthetic solute, net charge -2,
 PSE-3, --L 32 --N 256 (64 A box, h = 0.25 A), PR-SPC/E water.  |dmu| and
-max|dg| of FP32-only and FP32 + 3 FP64 polish steps against the FP64 solve;
-bar |dmu| < 0.01 kcal/mol.  There is no published mu for this case (parity
-of the FP64 path to the oracle is covered on smaller grids).
+max|dg| against the FP64 solve of: FP32 only (FP32 passes around the
+starting point until their floor; reported, no bar: the floor is the point
+of the study), and the mixed path (defect correction, then 3 FP64 polish
+steps; bar |dmu| < 0.01 kcal/mol).  There is no published mu for this case
+(parity of the FP64 path to the oracle is covered on smaller grids).
 
 Config 3 (gradients): one 30-site solute, HNC, --L 20 --N 128; adjoint
 d mu / d theta against central finite differences of the GPU mu (h = 1e-4)
@@ -37,18 +39,21 @@ def test_config2_precision_study(chi160):
     t64 = time.time() - t0
     assert f64["status"] == "OK"
     rows = []
-    for name, polish in (("fp32-only", 0), ("fp32+3xfp64", 3)):
+    for name, prec, polish in (("fp32-only", "fp32", 0), ("mixed+3xfp64", "mixed", 3)):
         t0 = time.time()
-        m = R.hnc3d_run_solute(sol, W, dict(s, precision="mixed", **{"polish-iters": polish,
-                                                                      "mixed-tol": 1e-5}),
+        m = R.hnc3d_run_solute(sol, W, dict(s, precision=prec, **{"polish-iters": polish,
+                                                                   "mixed-tol": 1e-5}),
                                chi=chi160)
         dt = time.time() - t0
         dmu = abs(m["free-energy"] - f64["free-energy"])
         dg = float(np.max(np.abs(m["GUV"] - f64["GUV"])))
         rows.append((name, dmu, dg))
         print(f"  {name}: mu {m['free-energy']:.6f} |dmu| {dmu:.2e} max|dg| {dg:.2e} "
-              f"iters {m['iterations']} (fp32 {m['iterations-fp32']}) {dt:.2f} s", flush=True)
-        assert dmu < 0.01
+              f"residual {m['residual']:.1e} iters {m['iterations']} "
+              f"(fp32 {m['iterations-fp32']}) {dt:.2f} s", flush=True)
+        if prec == "mixed":
+            assert m["status"] == "OK"
+            assert dmu < 0.01
     print(f"  fp64: mu {f64['free-energy']:.6f} iters {f64['iterations']} {t64:.2f} s", flush=True)
 
 
