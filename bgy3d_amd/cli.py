@@ -35,7 +35,8 @@ def parse(argv):
     ap.add_argument("--rmax", type=float)
     ap.add_argument("--nrad", type=int)
     ap.add_argument("--no-renorm", action="store_true")
-    ap.add_argument("--snes-solver", choices=["mdiis", "picard"], default="mdiis")
+    ap.add_argument("--snes-solver", choices=["mdiis", "picard", "newton", "jager", "trial"],
+                    default="mdiis")
     ap.add_argument("--lambda", dest="lam", type=float, default=0.02)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--chi-upscale", action="store_true")

@@ -122,6 +122,10 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
         else if (d.mode == 5)  // bv holds x_b (PY: exp(-beta v) - 1, c linear in t)
           c.v[k] = (g.closure == 2) ? b.v[k] * tv.v[k]
                                     : closure_dc<T>(g.closure, g.pse_order, b.v[k], tv.v[k]);
+        else if (d.mode == 6)  // J v: dc = (dc/dt) v (compute_c1, hnc3d.c:1307)
+          c.v[k] = T(d.dt[off + e + k] * (double)tv.v[k]);
+        else if (d.mode == 7)  // flag form: dc + v (hnc3d.c:1310-1320)
+          c.v[k] = T((d.dt[off + e + k] + 1.0) * (double)tv.v[k]);
         else c.v[k] = tv.v[k];
       }
     }
@@ -430,7 +434,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
     } else {
 #pragma unroll
       for (int k = 0; k < VW; ++k) {
-        if (mode == 0) rv.v[k] = z.v[k] - tv.v[k];                                  // F(t)
+        if (mode == 0 || mode == 6) rv.v[k] = z.v[k] - tv.v[k];                     // F(t), J v
         else if (mode == 1)                                                          // adjoint
           rv.v[k] = T(d.dt[off + e + k]) * z.v[k] + T(d.gv[off + e + k]) - tv.v[k];
         else rv.v[k] = z.v[k];                                                       // K t
@@ -964,6 +968,70 @@ grad_tau(GridArgs g, SitesDesc sd, const cpx<double>* __restrict__ Lam,
     for (int i = 1; i < NT / 64; ++i) v += red[threadIdx.x][i];
     part[((int64_t)a * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = -g.tau_scale * v;
   }
+}
+
+// ------------------------------------------------------ Newton-Krylov
+// dc/dt at t (closure1, closures.f90:131-153) for the matrix-free Newton
+// Jacobian, FP64, one solute.
+__global__ void __launch_bounds__(NT)
+closure_dcdt(GridArgs g, const double* __restrict__ t, const double* __restrict__ bv,
+             double* __restrict__ D, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) {
+    double ct, cv;
+    closure_derivs(g.closure, g.pse_order, bv[e], t[e], &ct, &cv);
+    D[e] = ct;
+  }
+}
+
+// Krylov helpers, FP64, one solute.  mdot: per workgroup partial sums of
+// <w, V_i>, V_i = V + i n for i < nv, to part[block * nv + i] (the host adds
+// the partials in a fixed order).  maxpy: w -= sum_i h_i V_i.  vcomb:
+// y = a x + b y.
+constexpr int KMAX = 32;
+__global__ void __launch_bounds__(NT)
+mdot(const double* __restrict__ V, int nv, const double* __restrict__ w, int64_t n,
+     double* __restrict__ part) {
+  __shared__ double red[KMAX][NT / 64];
+  double acc[KMAX];
+#pragma unroll
+  for (int i = 0; i < KMAX; ++i) acc[i] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) {
+    const double we = w[e];
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i)
+      if (i < nv) acc[i] += we * V[(int64_t)i * n + e];
+  }
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < KMAX; ++i)
+    if (i < nv) {
+      const double v = warp_sum(acc[i]);
+      if (lane == 0) red[i][wv] = v;
+    }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double v = red[threadIdx.x][0];
+    for (int q = 1; q < NT / 64; ++q) v += red[threadIdx.x][q];
+    part[(int64_t)blockIdx.x * nv + threadIdx.x] = v;
+  }
+}
+__global__ void __launch_bounds__(NT)
+maxpy(double* __restrict__ w, const double* __restrict__ V, const double* __restrict__ h, int nv,
+      int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) {
+    double acc = w[e];
+    for (int i = 0; i < nv; ++i) acc -= h[i] * V[(int64_t)i * n + e];
+    w[e] = acc;
+  }
+}
+__global__ void __launch_bounds__(NT)
+vcomb(double* __restrict__ y, const double* __restrict__ x, double a, double b, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride)
+    y[e] = a * x[e] + (b == 0.0 ? 0.0 : b * y[e]);
 }
 
 }  // namespace rism

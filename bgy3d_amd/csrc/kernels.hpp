@@ -34,8 +34,10 @@ struct SoluteDesc {
   int mode;                  // 0 F(t); 1 adjoint D K l + g - l; 2 K l; 3 spectrum;
                              // 5 defect correction: F(t_b + delta) for the
                              //   correction delta (t) around a base t_b, as
-                             //   F(t_b) + K (c(t_b + delta) - c(t_b)) - delta
-  const double* dt;          // mode 1: D = dc/dt at the converged t (FP64)
+                             //   F(t_b) + K (c(t_b + delta) - c(t_b)) - delta;
+                             // 6 J v = K (D v) - v; 7 J' v = K ((D + 1) v)
+                             //   (D = dc/dt in dt, jacobian_t1 and its flag form)
+  const double* dt;          // modes 1, 6, 7: D = dc/dt (FP64)
   const double* gv;          // mode 1: g = dmu/dt
   const void* fb;            // mode 5: F(t_b); bv then holds x_b = -beta v + t_b
                              //   (PY: exp(-beta v) - 1)

@@ -719,6 +719,232 @@ void iterate_batch(Problem& pb, const Config& cf) {
   HIPCHK(hipStreamSynchronize(pb.s));
 }
 
+// ------------------------------------------------ the reference's solvers
+// Newton (bgy3d_snes_newton, bgy3d-snes.c:161-346), Jager
+// (bgy3d_snes_jager, :402-519) and trial (:387-399), FP64, one solute at a
+// time (the others inactive).  Each keeps its state in t64 of the solute and
+// counts every full pass (residual or Jacobian application) in iters.
+struct OneSolute {
+  Problem& pb;
+  int b;
+  std::vector<SoluteDesc> sd;
+  std::vector<double> out;
+  OneSolute(Problem& p, int bb) : pb(p), b(bb), sd(p.B), out((size_t)p.B * NPART) {}
+  SoluteDesc& desc() {
+    for (auto& d : sd) std::memset(&d, 0, sizeof d);
+    auto& s = pb.st[b];
+    SoluteDesc& d = sd[b];
+    d.active = 1;
+    d.bv = s.bv64.p;
+    d.W = s.W.p;
+    d.Fs = s.fs64.p;
+    d.part = s.part.p;
+    return d;
+  }
+  // evaluate the prepared descriptor; returns (max|r|, ||r||^2)
+  std::pair<double, double> run() {
+    evaluate(pb, false, sd);
+    HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+    HIPCHK(hipGetLastError());
+    pb.st[b].iters++;
+    return {out[(size_t)b * NPART], out[(size_t)b * NPART + 1]};
+  }
+  // F at t = u0 + a u1 (ncomb 2) or at t itself (u1 == nullptr) into r;
+  // t64 receives the point
+  std::pair<double, double> F(const double* u0, const double* u1, double a, double* r) {
+    SoluteDesc& d = desc();
+    d.t = pb.st[b].t64.p;
+    if (u1) {
+      d.ncomb = 2;
+      d.u[0] = u0;
+      d.u[1] = u1;
+      d.coef[0] = 1.0;
+      d.coef[1] = a;
+    }
+    d.r = r;
+    return run();
+  }
+};
+
+// Picard x += lam F(x) with the step schedule of bgy3d_snes_jager when
+// jager, else fixed lam; at most nsteps residuals.  Returns true when
+// max|F| <= tol.
+bool picard_steps(Problem& pb, const Config& cf, int b, double lam, int nsteps, bool jager) {
+  OneSolute os(pb, b);
+  auto& s = pb.st[b];
+  DevBuf<double> r(pb.mP);
+  double a0 = lam, a1 = lam, a_prev = 0.0, norm_old = 0.0;
+  int mycount = 0, upwards = 0;
+  const int nth = 10;
+  for (int iter = 0; iter < nsteps; ++iter) {
+    auto [rinf, r2] = os.F(s.t64.p, iter ? r.p : nullptr, a_prev, r.p);
+    s.resid = rinf;
+    if (cf.verbose)
+      std::fprintf(stderr, "rism3d: solute %d iter %d %s |F|inf = %.3e step %.3g\n", b, s.iters,
+                   jager ? "jager" : "picard", rinf, a_prev);
+    if (!std::isfinite(rinf) || !std::isfinite(r2)) { s.status = RISM3D_NAN; return false; }
+    if (rinf <= cf.tol) { s.status = RISM3D_OK; return true; }
+    if (!jager) { a_prev = lam; continue; }
+    const double a = (iter % nth == 0) ? a1 : a0;
+    const bool up = rinf > norm_old;
+    norm_old = rinf;
+    mycount++;
+    if (iter % nth != 1) upwards = up ? 1 : 0;
+    if (iter % nth == 1 && up && upwards == 0 && iter > 2 * nth) {
+      a1 = std::max(a1 / 2.0, a0);
+      mycount = 0;
+    }
+    if (mycount > 2 * nth) {
+      a1 = std::min(a1 * 2.0, 1.0);
+      mycount = 0;
+    }
+    a_prev = a;
+  }
+  // the last step is applied, as the reference's loop does
+  if (nsteps > 0 && a_prev != 0.0) {
+    const int ch = (int)std::min<int64_t>(2048, (pb.mP + NT - 1) / NT);
+    hipLaunchKernelGGL(vcomb, dim3(ch), dim3(NT), 0, pb.s, s.t64.p, r.p, a_prev, 1.0, pb.mP);
+    // vcomb computes y = a x + b y: t64 = a_prev r + t64
+    HIPCHK(hipStreamSynchronize(pb.s));
+  }
+  s.status = RISM3D_NONCONVERGED;
+  return false;
+}
+
+// Newton with a matrix-free Jacobian J v = K (D v) - v (modes 6 and the
+// dc/dt field D), GMRES(30) with classical Gram-Schmidt applied twice, the
+// linear solve to 1e-3 relative (the reference's KSP rtol, 100 x PETSc's
+// 1e-5, bgy3d-snes.c:274), and a backtracking line search on ||F||_2
+// (SNESNEWTONLS: sufficient decrease 1e-4, halving, 10 tries).
+void newton_solve(Problem& pb, const Config& cf, int b, int max_newton) {
+  const int R = 30;
+  const double ksp_rtol = 1e-3;
+  auto& s = pb.st[b];
+  const int64_t n = pb.mP;
+  const int ch = (int)std::min<int64_t>(1024, (n + NT - 1) / NT);
+  OneSolute os(pb, b);
+  DevBuf<double> V((size_t)(R + 1) * n), w(n), r(n), t0(n), dx(n), D(n), part((size_t)ch * KMAX),
+      hdev(KMAX);
+  std::vector<double> hp((size_t)ch * KMAX);
+  auto dots = [&](const double* Vb, int nv, const double* wv, std::vector<double>& res) {
+    hipLaunchKernelGGL(mdot, dim3(ch), dim3(NT), 0, pb.s, Vb, nv, wv, n, part.p);
+    HIPCHK(hipMemcpyAsync(hp.data(), part.p, (size_t)ch * nv * 8, hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+    res.assign(nv, 0.0);
+    for (int c = 0; c < ch; ++c)
+      for (int i = 0; i < nv; ++i) res[i] += hp[(size_t)c * nv + i];
+  };
+  auto axpy_many = [&](double* wv, const double* Vb, const std::vector<double>& h, int nv) {
+    HIPCHK(hipMemcpy(hdev.p, h.data(), nv * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(maxpy, dim3(ch), dim3(NT), 0, pb.s, wv, Vb, hdev.p, nv, n);
+  };
+  auto [rinf, r2] = os.F(s.t64.p, nullptr, 0.0, r.p);
+  for (int it = 0;; ++it) {
+    s.resid = rinf;
+    if (cf.verbose)
+      std::fprintf(stderr, "rism3d: solute %d newton %d |F|inf = %.3e |F|2 = %.3e\n", b, it, rinf,
+                   std::sqrt(r2));
+    if (!std::isfinite(rinf) || !std::isfinite(r2)) { s.status = RISM3D_NAN; return; }
+    if (rinf <= cf.tol) { s.status = RISM3D_OK; return; }
+    if (it >= max_newton || s.iters >= cf.max_iter * 40) { s.status = RISM3D_NONCONVERGED; return; }
+    // D = dc/dt at t; GMRES on J dx = -r from dx = 0
+    hipLaunchKernelGGL(closure_dcdt, dim3(ch), dim3(NT), 0, pb.s, pb.g, s.t64.p, s.bv64.p, D.p, n);
+    const double beta = std::sqrt(r2);
+    hipLaunchKernelGGL(vcomb, dim3(ch), dim3(NT), 0, pb.s, V.p, r.p, -1.0 / beta, 0.0, n);
+    std::vector<std::vector<double>> H(R + 1, std::vector<double>(R, 0.0));
+    std::vector<double> cs(R, 0.0), sn(R, 0.0), gg(R + 1, 0.0);
+    gg[0] = beta;
+    int k = 0;
+    for (int j = 0; j < R; ++j) {
+      SoluteDesc& d = os.desc();
+      d.t = V.p + (size_t)j * n;
+      d.r = w.p;
+      d.dt = D.p;
+      d.mode = 6;
+      os.run();
+      std::vector<double> h, h2;
+      dots(V.p, j + 1, w.p, h);
+      axpy_many(w.p, V.p, h, j + 1);
+      dots(V.p, j + 1, w.p, h2);
+      axpy_many(w.p, V.p, h2, j + 1);
+      for (int i = 0; i <= j; ++i) H[i][j] = h[i] + h2[i];
+      std::vector<double> nn;
+      dots(w.p, 1, w.p, nn);
+      const double hn = std::sqrt(std::max(nn[0], 0.0));
+      H[j + 1][j] = hn;
+      if (hn > 0)
+        hipLaunchKernelGGL(vcomb, dim3(ch), dim3(NT), 0, pb.s, V.p + (size_t)(j + 1) * n, w.p,
+                           1.0 / hn, 0.0, n);
+      for (int i = 0; i < j; ++i) {  // earlier rotations
+        const double a = H[i][j], c = H[i + 1][j];
+        H[i][j] = cs[i] * a + sn[i] * c;
+        H[i + 1][j] = -sn[i] * a + cs[i] * c;
+      }
+      const double den = std::hypot(H[j][j], H[j + 1][j]);
+      cs[j] = den > 0 ? H[j][j] / den : 1.0;
+      sn[j] = den > 0 ? H[j + 1][j] / den : 0.0;
+      H[j][j] = den;
+      H[j + 1][j] = 0.0;
+      gg[j + 1] = -sn[j] * gg[j];
+      gg[j] = cs[j] * gg[j];
+      k = j + 1;
+      if (std::fabs(gg[j + 1]) <= ksp_rtol * beta || hn == 0.0) break;
+    }
+    std::vector<double> y(k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+      double acc = gg[i];
+      for (int q = i + 1; q < k; ++q) acc -= H[i][q] * y[q];
+      y[i] = H[i][i] != 0.0 ? acc / H[i][i] : 0.0;
+    }
+    // dx = sum_i y_i V_i
+    hipLaunchKernelGGL(vcomb, dim3(ch), dim3(NT), 0, pb.s, dx.p, V.p, y[0], 0.0, n);
+    if (k > 1) {
+      std::vector<double> my(k - 1);
+      for (int i = 1; i < k; ++i) my[i - 1] = -y[i];
+      axpy_many(dx.p, V.p + n, my, k - 1);
+    }
+    // line search from t0 = t
+    HIPCHK(hipMemcpyAsync(t0.p, s.t64.p, n * 8, hipMemcpyDeviceToDevice, pb.s));
+    double alpha = 1.0;
+    bool accepted = false;
+    for (int ls = 0; ls < 10; ++ls) {
+      auto [ri, rr] = os.F(t0.p, dx.p, alpha, r.p);
+      if (std::isfinite(ri) && std::isfinite(rr) && rr <= (1.0 - 2e-4 * alpha) * r2) {
+        rinf = ri;
+        r2 = rr;
+        accepted = true;
+        break;
+      }
+      alpha *= 0.5;
+    }
+    if (!accepted) {
+      HIPCHK(hipMemcpyAsync(s.t64.p, t0.p, n * 8, hipMemcpyDeviceToDevice, pb.s));
+      HIPCHK(hipStreamSynchronize(pb.s));
+      s.status = RISM3D_NONCONVERGED;  // line search failed (SNES would diverge)
+      return;
+    }
+  }
+}
+
+void reference_solver(Problem& pb, const Config& cf) {
+  for (int b = 0; b < pb.B; ++b) {
+    auto& s = pb.st[b];
+    if (!s.active) continue;
+    if (cf.solver == RISM3D_SOLVER_JAGER) {
+      picard_steps(pb, cf, b, cf.lam, cf.max_iter, true);
+    } else {
+      bool done = false;
+      if (cf.solver == RISM3D_SOLVER_TRIAL) {
+        Config c = cf;
+        done = picard_steps(pb, c, b, 0.02, 100, false);
+      }
+      if (!done && s.status != RISM3D_NAN) newton_solve(pb, cf, b, cf.max_iter);
+    }
+    s.active = 0;
+  }
+}
+
 // Closure continuation for solutes whose iteration diverged (max|F| not
 // finite): HNC and PSE-n (n >= 2) grow like exp(x) or x^n where the solute
 // field is strongly attractive, and from t = 0 the first steps can
@@ -804,6 +1030,8 @@ struct Inputs {
   rism3d_options opt;
   const double* t_eval = nullptr;  // residual_only: t in
   double* r_out = nullptr;         // residual_only: F(t) out
+  const double* j_dt = nullptr;    // jacobian: dt in (t in t_eval), J dt out in r_out
+  int j_flag = 0;
 };
 
 int validate(const Inputs& in) {
@@ -828,6 +1056,13 @@ int validate(const Inputs& in) {
   const int imax = (int)(std::sqrt(k2) / dk - 0.5 + 0.5);
   if (imax > pd->nrad - 1) return RISM3D_BAD_ARG;
   if (in.opt.diis_depth < 1 || in.opt.diis_depth > MAX_DIIS) return RISM3D_BAD_ARG;
+  if (in.opt.solver < RISM3D_SOLVER_MDIIS || in.opt.solver > RISM3D_SOLVER_TRIAL)
+    return RISM3D_BAD_ARG;
+  if (in.opt.solver >= RISM3D_SOLVER_NEWTON && in.opt.precision != RISM3D_FP64)
+    return RISM3D_UNSUPPORTED;
+  if ((in.opt.solver == RISM3D_SOLVER_NEWTON || in.opt.solver == RISM3D_SOLVER_TRIAL) &&
+      pd->closure == RISM3D_CLOSURE_PY)
+    return RISM3D_UNSUPPORTED;  // no closure differential for PY (closures.f90:131-153)
   // z-pass LDS: m padded lines per kx column, at least 2 columns
   {
     const size_t lds64 =
@@ -1037,7 +1272,7 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   const rism3d_problem& pd = *in.pd;
   Problem pb;
   pb.B = in.B;
-  pb.D = in.opt.solver == RISM3D_SOLVER_PICARD ? 1 : in.opt.diis_depth;
+  pb.D = in.opt.solver == RISM3D_SOLVER_MDIIS ? in.opt.diis_depth : 1;
   pb.mixed = in.opt.precision != RISM3D_FP64;
   for (int d = 0; d < 3; ++d) { pb.N[d] = pd.N[d]; pb.L[d] = pd.L[d]; pb.h[d] = pd.L[d] / pd.N[d]; }
   GridArgs& g = pb.g;
@@ -1189,6 +1424,34 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     HIPCHK(hipGetLastError());
   }
 
+  if (in.j_dt) {
+    // one Jacobian application at t (jacobian_t1, hnc3d.c:1290-1362)
+    auto& s0 = pb.st[0];
+    const int64_t mP = pb.mP;
+    const int ch = (int)std::min<int64_t>(2048, (mP + NT - 1) / NT);
+    DevBuf<double> D(mP), v(mP), jv(mP);
+    HIPCHK(hipMemcpyAsync(s0.t64.p, in.t_eval, mP * 8, hipMemcpyHostToDevice, pb.s));
+    HIPCHK(hipMemcpyAsync(v.p, in.j_dt, mP * 8, hipMemcpyHostToDevice, pb.s));
+    hipLaunchKernelGGL(closure_dcdt, dim3(ch), dim3(NT), 0, pb.s, g, s0.t64.p, s0.bv64.p, D.p, mP);
+    std::vector<SoluteDesc> sd(1);
+    std::memset(sd.data(), 0, sizeof(SoluteDesc));
+    sd[0].t = v.p;
+    sd[0].r = jv.p;
+    sd[0].bv = s0.bv64.p;
+    sd[0].W = s0.W.p;
+    sd[0].Fs = s0.fs64.p;
+    sd[0].part = s0.part.p;
+    sd[0].dt = D.p;
+    sd[0].mode = in.j_flag ? 7 : 6;
+    sd[0].active = 1;
+    evaluate(pb, false, sd);
+    HIPCHK(hipMemcpyAsync(in.r_out, jv.p, mP * 8, hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+    HIPCHK(hipGetLastError());
+    res[0].status = RISM3D_OK;
+    return RISM3D_OK;
+  }
+
   if (in.t_eval) {
     // one residual evaluation F(t) = T(t) - t (iterate_t1, hnc3d.c:1184)
     auto& s0 = pb.st[0];
@@ -1226,11 +1489,16 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   cf.depth = pb.D;
   cf.verbose = in.opt.verbose;
   cf.max_iter = pd.max_iter > 0 ? pd.max_iter : 1000;
-  cf.lam = in.opt.solver == RISM3D_SOLVER_PICARD ? pd.lambda : in.opt.diis_step;
+  cf.lam = (in.opt.solver == RISM3D_SOLVER_PICARD || in.opt.solver == RISM3D_SOLVER_JAGER)
+               ? pd.lambda : in.opt.diis_step;
   cf.tol = pd.norm_tol;
   cf.mixed_tol = in.opt.mixed_tol;
-  iterate_batch(pb, cf);
-  if (in.opt.continuation) continuation(pb, cf, pd.closure);
+  if (cf.solver >= RISM3D_SOLVER_NEWTON) {
+    reference_solver(pb, cf);
+  } else {
+    iterate_batch(pb, cf);
+    if (in.opt.continuation) continuation(pb, cf, pd.closure);
+  }
 
   // observables in FP64 from the converged t (hnc3d.c:2157-2268)
   {
@@ -1421,6 +1689,36 @@ int rism3d_residual(const rism3d_problem* pd, int m, const rism3d_site* solvent,
   if (opt) in.opt = *opt; else rism3d_options_default(&in.opt);
   in.t_eval = t;
   in.r_out = r;
+  rism3d_result res;
+  std::memset(&res, 0, sizeof res);
+  try {
+    return rism::run(in, &res, nullptr, nullptr);
+  } catch (const rism::HipFailure& e) {
+    return (e.err == hipErrorOutOfMemory) ? RISM3D_OOM : RISM3D_NO_DEVICE;
+  }
+}
+
+int rism3d_jacobian(const rism3d_problem* pd, int m, const rism3d_site* solvent, int n,
+                    const rism3d_site* solute, const double* chi_k, const rism3d_options* opt,
+                    const double* t, const double* dt, int flag, double* jdt) {
+  if (!pd || !chi_k || !t || !dt || !jdt || !solute) return RISM3D_BAD_ARG;
+  if (pd->closure == RISM3D_CLOSURE_PY) return RISM3D_UNSUPPORTED;
+  rism::Inputs in;
+  in.pd = pd;
+  in.m = m;
+  in.solvent = solvent;
+  in.B = 1;
+  const rism3d_site* sol[1] = {solute};
+  int ns[1] = {n};
+  in.n_sites = ns;
+  in.solutes = sol;
+  in.chi_k = chi_k;
+  if (opt) in.opt = *opt; else rism3d_options_default(&in.opt);
+  in.opt.precision = RISM3D_FP64;
+  in.t_eval = t;
+  in.j_dt = dt;
+  in.j_flag = flag ? 1 : 0;
+  in.r_out = jdt;
   rism3d_result res;
   std::memset(&res, 0, sizeof res);
   try {

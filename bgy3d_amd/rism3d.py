@@ -30,7 +30,7 @@ CLOSURES = {"HNC": 0, "KH": 1, "PY": 2, **{f"PSE{n}": 3 + n for n in range(8)}}
 OK, NONCONVERGED, NAN, BAD_ARG, NO_DEVICE, OOM, UNSUPPORTED = range(7)
 STATUS_NAMES = ["OK", "NONCONVERGED", "NAN", "BAD_ARG", "NO_DEVICE", "OOM", "UNSUPPORTED"]
 
-SOLVER_MDIIS, SOLVER_PICARD = 0, 1
+SOLVER_MDIIS, SOLVER_PICARD, SOLVER_NEWTON, SOLVER_JAGER, SOLVER_TRIAL = 0, 1, 2, 3, 4
 FP64, MIXED, FP32 = 0, 1, 2
 
 
@@ -82,8 +82,9 @@ class Result(ctypes.Structure):
 
 
 EXPORTS = ("rism3d_options_default", "rism3d_solute_solve", "rism3d_batch_solve",
-           "rism3d_residual", "rism3d_restart_destroy", "rism3d_solvent_chi",
-           "rism3d_version", "rism3d_profile_start", "rism3d_profile_read")
+           "rism3d_residual", "rism3d_jacobian", "rism3d_restart_destroy",
+           "rism3d_solvent_chi", "rism3d_version", "rism3d_profile_start",
+           "rism3d_profile_read")
 
 KERNELS = ("closure_r2c_x", "fft_y_fwd", "zstar", "fft_y_inv", "c2r_x_resid")
 
@@ -112,6 +113,9 @@ def load():
         lib.rism3d_residual.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_int,
                                         P(Site), dp, P(Options), dp, dp]
         lib.rism3d_residual.restype = ctypes.c_int
+        lib.rism3d_jacobian.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_int,
+                                        P(Site), dp, P(Options), dp, dp, ctypes.c_int, dp]
+        lib.rism3d_jacobian.restype = ctypes.c_int
         lib.rism3d_restart_destroy.argtypes = [ctypes.c_void_p]
         lib.rism3d_restart_destroy.restype = None
         lib.rism3d_solvent_chi.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_double, dp]
@@ -189,7 +193,8 @@ def make_options(settings=None, **kw):
     if s.get("no-renorm"):
         opt.renorm = 0
     if "snes-solver" in s:
-        opt.solver = {"mdiis": SOLVER_MDIIS, "picard": SOLVER_PICARD}[s["snes-solver"]]
+        opt.solver = {"mdiis": SOLVER_MDIIS, "picard": SOLVER_PICARD, "newton": SOLVER_NEWTON,
+                      "jager": SOLVER_JAGER, "trial": SOLVER_TRIAL}[s["snes-solver"]]
     for key, field in (("precision", "precision"), ("polish-iters", "polish_iters"),
                        ("mixed-tol", "mixed_tol"), ("diis-depth", "diis_depth"),
                        ("diis-step", "diis_step"), ("verbose", "verbose"),
@@ -338,6 +343,26 @@ def residual(solute, solvent, settings, chi, t):
     if st != OK:
         raise RismError(st, "rism3d_residual")
     return r
+
+
+def jacobian(solute, solvent, settings, chi, t, dt, flag=False):
+    """One application of the Jacobian at t (jacobian_t1, hnc3d.c:1290-1362):
+    (chi - 1) * (dc/dt dt) - dt, or with flag its flag form (chi - 1) *
+    (dc/dt dt + dt)."""
+    lib = load()
+    pd = problem_data(settings)
+    opt = make_options(settings)
+    sv = make_sites(solvent)
+    su = make_sites(solute)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    dt = np.ascontiguousarray(dt, dtype=np.float64)
+    out = np.empty_like(t)
+    chi = np.ascontiguousarray(chi, dtype=np.float64)
+    st = lib.rism3d_jacobian(ctypes.byref(pd), len(sv), sv, len(su), su, _dptr(chi),
+                             ctypes.byref(opt), _dptr(t), _dptr(dt), int(bool(flag)), _dptr(out))
+    if st != OK:
+        raise RismError(st, "rism3d_jacobian")
+    return out
 
 
 def rism_solvent(solvent, settings):

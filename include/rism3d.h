@@ -78,9 +78,19 @@ enum rism3d_status {
   RISM3D_UNSUPPORTED = 6    /* grid size or site count outside limits   */
 };
 
+/* The reference's snes-solver choices (bgy3d_snes_default,
+   bgy3d-snes.c:16-35) plus MDIIS.  Every solver stops at max|F| <=
+   norm_tol.  NEWTON, JAGER and TRIAL run in FP64 only. */
 enum rism3d_solver {
   RISM3D_SOLVER_MDIIS = 0,  /* default: MDIIS / Anderson mixing         */
-  RISM3D_SOLVER_PICARD = 1  /* x += lambda F(x) (bgy3d-snes.c:348-384)  */
+  RISM3D_SOLVER_PICARD = 1, /* x += lambda F(x) (bgy3d-snes.c:348-384)  */
+  RISM3D_SOLVER_NEWTON = 2, /* Newton, GMRES(30) on the matrix-free
+                               Jacobian, backtracking line search
+                               (bgy3d_snes_newton, bgy3d-snes.c:161-346) */
+  RISM3D_SOLVER_JAGER = 3,  /* Picard with the adaptive step schedule of
+                               bgy3d_snes_jager (bgy3d-snes.c:402-519)   */
+  RISM3D_SOLVER_TRIAL = 4   /* 100 Picard steps at lambda = 0.02, then
+                               Newton (bgy3d_snes_trial, :387-399)       */
 };
 
 enum rism3d_precision {
@@ -202,6 +212,20 @@ int rism3d_residual(const rism3d_problem *pd,
                     const double *chi_k,
                     const rism3d_options *opt,
                     const double *t, double *r);
+
+/* One Jacobian application at t, replacing jacobian_t1
+   (hnc3d.c:1290-1362): jdt = (chi - 1) * (dc/dt dt) - dt, or with
+   flag != 0 its flag form (chi - 1) * (dc/dt dt + dt) that the response
+   path uses (hnc3d.c:1310-1320, 1387-1391).  FP64; chi_k required;
+   RISM3D_UNSUPPORTED for PY, whose differential the reference lacks
+   (closures.f90:131-153).  t, dt, jdt: double[m][N2][N1][N0]. */
+int rism3d_jacobian(const rism3d_problem *pd,
+                    int m, const rism3d_site *solvent,
+                    int n, const rism3d_site *solute,
+                    const double *chi_k,
+                    const rism3d_options *opt,
+                    const double *t, const double *dt, int flag,
+                    double *jdt);
 
 /* Replaces bgy3d_restart_destroy (bgy3d-impure.c:593-600). */
 void rism3d_restart_destroy(rism3d_restart *restart);

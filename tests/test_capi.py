@@ -105,6 +105,21 @@ def test_validation_before_device():
     assert e.value.status == R.BAD_ARG
     with pytest.raises(ValueError):
         R.hnc3d_run_solute(W, W, base, chi=_chi(nrad=100))
+    # the reference's Newton / trial / Jager solvers run in FP64 only, and
+    # Newton needs the closure differential PY lacks (closures.f90:131-153)
+    for solver in ("newton", "jager", "trial"):
+        with pytest.raises(R.RismError) as e:
+            R.hnc3d_run_solute(W, W, dict(base, **{"snes-solver": solver, "precision": "mixed"}),
+                               chi=_chi())
+        assert e.value.status == R.UNSUPPORTED
+    with pytest.raises(R.RismError) as e:
+        R.hnc3d_run_solute(W, W, dict(base, closure="PY", **{"snes-solver": "newton"}),
+                           chi=_chi())
+    assert e.value.status == R.UNSUPPORTED
+    with pytest.raises(R.RismError) as e:
+        R.jacobian(W, W, dict(base, closure="PY"), _chi(), np.zeros((3, 32, 32, 32)),
+                   np.zeros((3, 32, 32, 32)))
+    assert e.value.status == R.UNSUPPORTED
 
 
 @pytest.mark.skipif(_gpu_present(), reason="checks the no-GPU error path")

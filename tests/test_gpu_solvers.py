@@ -1,0 +1,69 @@
+"""The reference's solver choices on the GPU (bgy3d_snes_default,
+bgy3d-snes.c:16-35) and the forward Jacobian they use.
+
+* rism3d_jacobian (jacobian_t1, hnc3d.c:1290-1362, and its flag form used
+  by the response path) vs the oracle's Solute3D.jacobian on seeded t, dt:
+  1e-9 absolute, as the residual parity.
+* Newton-GMRES, Jager and trial reach the MDIIS fixed point: |dmu| < 1e-9,
+  max|dg| < 1e-8 at max|F| <= 1e-11.
+"""
+import numpy as np
+import pytest
+
+from conftest import README
+from oracle import closures as C
+from oracle import rism3d as O
+
+pytestmark = pytest.mark.gpu
+W = "water, PR-SPC/E"
+
+
+def _settings(closure="HNC", **kw):
+    return dict(README, N=32, L=8.0, rmax=40.0, nrad=1536, closure=closure, **kw)
+
+
+def _smooth(pd, m, seed, amp=1.0):
+    rng = np.random.default_rng(seed)
+    x, y, z = O.r_axes(pd)
+    r2 = z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2
+    t = np.stack([amp * rng.uniform(-1, 1) * np.exp(-r2 / rng.uniform(4, 16)) for _ in range(m)])
+    return t + 0.01 * amp * rng.standard_normal(t.shape)
+
+
+@pytest.mark.parametrize("closure,solute", [("HNC", W), ("KH", "Na+, JC08, SPC/E"),
+                                            ("PSE3", W)])
+@pytest.mark.parametrize("flag", [0, 1])
+def test_jacobian_vs_oracle(closure, solute, flag, chi_w):
+    print(f"case jacobian {closure} {solute} flag={flag}", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd.solutes import find_molecule
+    s = _settings(closure)
+    pd = O.Problem.from_cli(8.0, 32, README["beta"], README["rho"], closure=C.NAMES[closure],
+                            rmax=40.0, nrad=1536)
+    ref = O.Solute3D(pd, find_molecule(W), find_molecule(solute), chi_w)
+    t = _smooth(pd, 3, 11)
+    dt = _smooth(pd, 3, 12, amp=0.1)
+    want = ref.jacobian(t, dt, flag=bool(flag))
+    got = R.jacobian(solute, W, s, chi_w, t, dt, flag=bool(flag))
+    err = np.max(np.abs(got - want))
+    print(f"  max|dJ| = {err:.3e}  max|J| = {np.max(np.abs(want)):.3e}", flush=True)
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("solver", ["newton", "trial", "jager", "picard"])
+@pytest.mark.parametrize("closure,solute", [("HNC", W), ("KH", "Na+, JC08, SPC/E")])
+def test_reference_solvers_reach_the_mdiis_root(solver, closure, solute, chi_w):
+    print(f"case solver {solver} {closure} {solute}", flush=True)
+    from bgy3d_amd import rism3d as R
+    s = _settings(closure, **{"norm-tol": 1e-11, "max-iter": 20000})
+    ref = R.hnc3d_run_solute(solute, W, s, chi=chi_w)
+    lam = {"picard": 0.3, "jager": 0.1}.get(solver, 0.02)
+    out = R.hnc3d_run_solute(solute, W, dict(s, **{"snes-solver": solver, "lambda": lam}),
+                             chi=chi_w)
+    dmu = abs(out["free-energy"] - ref["free-energy"])
+    dg = np.max(np.abs(out["GUV"] - ref["GUV"]))
+    print(f"  {out['status']} passes {out['iterations']} (mdiis {ref['iterations']}) "
+          f"|dmu| {dmu:.2e} max|dg| {dg:.2e}", flush=True)
+    assert out["status"] == "OK"
+    assert dmu < 1e-9 and dg < 1e-8
+
