@@ -250,13 +250,26 @@ def irfft(pd, a):
 class Solute3D:
     """Everything hnc3d_solute_solve sets up before SNESSolve."""
 
-    def __init__(self, pd, solvent, solute, chi_1d, renorm=True):
+    def __init__(self, pd, solvent, solute, chi_1d, renorm=True, chi3d=None):
+        """chi3d: None, or chi - 1 of a 3D solvent solve on this grid,
+        [m(m+1)/2][N2][N1][N0/2+1] by pair i(i+1)/2 + j (j <= i): the
+        --solvent-3d path (solvent_kernel_file, hnc3d.c:1434-1437 and
+        1572-1590), which has no tau renormalisation."""
         self.pd = pd
         self.solvent = solvent
         self.solute = solute
         self.m = len(solvent)
         self.q = np.array([s[4] for s in solvent])
-        self.chi, tau = solvent_kernel_rism(pd, solvent, chi_1d)
+        if chi3d is not None:
+            m = self.m
+            self.chi = np.empty((m, m) + chi3d.shape[1:])
+            for i in range(m):
+                for j in range(i + 1):
+                    self.chi[i, j] = self.chi[j, i] = chi3d[i * (i + 1) // 2 + j]
+            tau = None
+            renorm = False
+        else:
+            self.chi, tau = solvent_kernel_rism(pd, solvent, chi_1d)
         self.tau_k = tau                    # tau_i(|k|) before F(k) s(k)
         self.renorm = renorm
         if renorm:
