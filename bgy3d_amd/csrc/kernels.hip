@@ -126,6 +126,8 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
           c.v[k] = T(d.dt[off + e + k] * (double)tv.v[k]);
         else if (d.mode == 7)  // flag form: dc + v (hnc3d.c:1310-1320)
           c.v[k] = T((d.dt[off + e + k] + 1.0) * (double)tv.v[k]);
+        else if (d.mode == 10)  // rho h for the susceptibility (hnc3d.c:1006-1030)
+          c.v[k] = T(g.rho) * (closure_c<T>(g.closure, g.pse_order, b.v[k], tv.v[k]) + tv.v[k]);
         else c.v[k] = tv.v[k];
       }
     }
@@ -290,9 +292,130 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   }
   const bool linear = d.mode != 0;  // chi convolution only
 
-  const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + base;
   const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
   const double ky = iy * g.dky;
+  if (g.special != 0) {
+    for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
+      const int z = e / LINES, l = e - z * LINES;
+      if (kx0 + l >= g.NX) continue;
+      const int64_t kk = base + z * zs + l;
+      const int iz = (z <= N2 / 2) ? z : z - N2;
+      const int ix = kx0 + l;
+      const double sgn = ((ix + iy + iz) & 1) ? -1.0 : 1.0;
+      if (g.special == 2) {
+        // Poisson: u(k) = 4 pi/eps0 h^3 rho(k) / k^2, 0 at k = 0, k^2 summed
+        // z, y, x (bgy3d_poisson, bgy3d-poisson.c:103-190); / L^3 for the
+        // inverse FFT
+        const double kz = iz * g.dkz, kx = ix * g.dkx;
+        const double k2 = (kz * kz + ky * ky) + kx * kx;
+        const double fac = k2 != 0.0 ? 4 * M_PI * EPSILON0INV * g.h3 / k2 * g.inv_L3 : 0.0;
+        for (int s = 0; s < m; ++s) {
+          cpx<T>& v = lds[(s * LINES + l) * LS + lpos(z)];
+          v = cscale(v, T(fac));
+        }
+        continue;
+      }
+      // unit long-range Coulomb as vec_kmap tabulates it: k = |k|, then
+      // 4 pi/eps0 exp(-k^2/4G^2)/k^2 (bgy3d-force.c:19-36, bgy3d-force.h)
+      const double kabs = sqrt(kmag2(ix * g.dkx, ky, iz * g.dkz));
+      const double k2 = kabs * kabs;
+      const double U = k2 == 0.0 ? 0.0
+          : (4 * M_PI * EPSILON0INV) * exp(-k2 / (4 * G_COULOMB * G_COULOMB)) / k2;
+      if (g.special == 3) {
+        for (int s = 0; s < m; ++s)
+          lds[(s * LINES + l) * LS + lpos(z)] = cpx<T>{T(U * sgn * g.inv_L3), T(0)};
+        continue;
+      }
+      // solvent-solvent OZ (iterate_t2, hnc3d.c:376-470; compute_t2_m,
+      // hnc3d.c:231-324): C = h^3 c s - beta q_i q_j U at the corner,
+      // X = (1 - rho W C)^-1 W C W, T = ((X - C) - beta q_i q_j U) s / L^3
+      const int ms = g.msol;
+      double Cr[3][3], Ci[3][3], Wm[3][3];
+      for (int i = 0; i < ms; ++i)
+        for (int j = 0; j <= i; ++j) {
+          const int p = i * (i + 1) / 2 + j;
+          const cpx<T> c = lds[(p * LINES + l) * LS + lpos(z)];
+          const double q2 = g.q[i] * g.q[j];
+          const double re = (double)c.re * g.h3 * sgn - g.beta * q2 * U;
+          const double im = (double)c.im * g.h3 * sgn;
+          Cr[i][j] = Cr[j][i] = re;
+          Ci[i][j] = Ci[j][i] = im;
+          Wm[i][j] = Wm[j][i] = (i == j) ? 1.0 : (double)chi3[p * g.K + kk];
+        }
+      // WC, A = 1 - rho WC, B = WC W
+      double WCr[3][3], WCi[3][3], Ar[3][3], Ai[3][3], Br[3][3], Bi[3][3];
+      for (int i = 0; i < ms; ++i)
+        for (int j = 0; j < ms; ++j) {
+          double ar = 0.0, ai = 0.0;
+          for (int q = 0; q < ms; ++q) { ar += Wm[i][q] * Cr[q][j]; ai += Wm[i][q] * Ci[q][j]; }
+          WCr[i][j] = ar;
+          WCi[i][j] = ai;
+          Ar[i][j] = (i == j ? 1.0 : 0.0) - g.rho * ar;
+          Ai[i][j] = -g.rho * ai;
+        }
+      for (int i = 0; i < ms; ++i)
+        for (int j = 0; j < ms; ++j) {
+          double br = 0.0, bi = 0.0;
+          for (int q = 0; q < ms; ++q) { br += WCr[i][q] * Wm[q][j]; bi += WCi[i][q] * Wm[q][j]; }
+          Br[i][j] = br;
+          Bi[i][j] = bi;
+        }
+      // Gaussian elimination with partial pivoting, A X = B (ms <= 3)
+      for (int col = 0; col < ms; ++col) {
+        int piv = col;
+        double best = Ar[col][col] * Ar[col][col] + Ai[col][col] * Ai[col][col];
+        for (int r = col + 1; r < ms; ++r) {
+          const double a2 = Ar[r][col] * Ar[r][col] + Ai[r][col] * Ai[r][col];
+          if (a2 > best) { best = a2; piv = r; }
+        }
+        if (piv != col)
+          for (int q = 0; q < ms; ++q) {
+            double tr = Ar[col][q], ti = Ai[col][q];
+            Ar[col][q] = Ar[piv][q]; Ai[col][q] = Ai[piv][q]; Ar[piv][q] = tr; Ai[piv][q] = ti;
+            tr = Br[col][q]; ti = Bi[col][q];
+            Br[col][q] = Br[piv][q]; Bi[col][q] = Bi[piv][q]; Br[piv][q] = tr; Bi[piv][q] = ti;
+          }
+        const double dr = Ar[col][col], di = Ai[col][col];
+        const double den = dr * dr + di * di;
+        const double ir = dr / den, ii = -di / den;  // 1 / A[col][col]
+        for (int r = col + 1; r < ms; ++r) {
+          const double fr = Ar[r][col] * ir - Ai[r][col] * ii;
+          const double fi = Ar[r][col] * ii + Ai[r][col] * ir;
+          for (int q = col; q < ms; ++q) {
+            Ar[r][q] -= fr * Ar[col][q] - fi * Ai[col][q];
+            Ai[r][q] -= fr * Ai[col][q] + fi * Ar[col][q];
+          }
+          for (int q = 0; q < ms; ++q) {
+            Br[r][q] -= fr * Br[col][q] - fi * Bi[col][q];
+            Bi[r][q] -= fr * Bi[col][q] + fi * Br[col][q];
+          }
+        }
+      }
+      for (int col = ms - 1; col >= 0; --col) {
+        const double dr = Ar[col][col], di = Ai[col][col];
+        const double den = dr * dr + di * di;
+        const double ir = dr / den, ii = -di / den;
+        for (int q = 0; q < ms; ++q) {
+          double xr = Br[col][q], xi = Bi[col][q];
+          for (int r = col + 1; r < ms; ++r) {
+            xr -= Ar[col][r] * Br[r][q] - Ai[col][r] * Bi[r][q];
+            xi -= Ar[col][r] * Bi[r][q] + Ai[col][r] * Br[r][q];
+          }
+          Br[col][q] = xr * ir - xi * ii;
+          Bi[col][q] = xr * ii + xi * ir;
+        }
+      }
+      for (int i = 0; i < ms; ++i)
+        for (int j = 0; j <= i; ++j) {
+          const int p = i * (i + 1) / 2 + j;
+          const double q2 = g.q[i] * g.q[j];
+          const double tr = ((Br[i][j] - Cr[i][j]) - g.beta * q2 * U) * sgn * g.inv_L3;
+          const double ti = (Bi[i][j] - Ci[i][j]) * sgn * g.inv_L3;
+          lds[(p * LINES + l) * LS + lpos(z)] = cpx<T>{T(tr), T(ti)};
+        }
+    }
+  } else {
+  const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + base;
   for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
     const int z = e / LINES, l = e - z * LINES;
     if (kx0 + l >= g.NX) continue;
@@ -335,6 +458,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
       lds[(i * LINES + l) * LS + lpos(z)] = out;
     }
   }
+  }  // g.special
   __syncthreads();
   for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, true>(lds + s * LINES * LS, LS, nl, tw);
   for (int s = 0; s < m; ++s) {
@@ -1032,6 +1156,153 @@ vcomb(double* __restrict__ y, const double* __restrict__ x, double a, double b, 
   const int64_t stride = (int64_t)gridDim.x * NT;
   for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride)
     y[e] = a * x[e] + (b == 0.0 ? 0.0 : b * y[e]);
+}
+
+// ------------------------------------------------ 3D solvent-solvent
+// beta v_short of every site pair around the grid centre
+// (bgy3d_pair_potential, bgy3d-force.c:123-143): LJ with its epsilon *
+// 1e8 ceiling and erfc Coulomb with its 1e3 ceiling (bgy3d-force.h:23-88),
+// r^2 summed x, y, z (vec_rmap, bgy3d-vec.h:639-648).  pse: [msol][3]
+// sigma, epsilon, charge.  Grid: x = chunks of P, y = pair.
+__global__ void __launch_bounds__(NT)
+pair_field(GridArgs g, const double* __restrict__ sse, double L0, double L1, double L2,
+           double h0, double h1, double h2, double* __restrict__ bv) {
+  const int p = blockIdx.y;
+  int i = 0;
+  while ((i + 1) * (i + 2) / 2 <= p) ++i;
+  const int j = p - i * (i + 1) / 2;
+  const double eps = sqrt(sse[3 * i + 1] * sse[3 * j + 1]);
+  const double sig = 0.5 * (sse[3 * i] + sse[3 * j]);
+  const double q2 = sse[3 * i + 2] * sse[3 * j + 2];
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.P; e += stride) {
+    const int x = (int)(e % g.N0);
+    const int y = (int)((e / g.N0) % g.N1);
+    const int z = (int)(e / ((int64_t)g.N0 * g.N1));
+    const double rx = x * h0 - L0 / 2, ry = y * h1 - L1 / 2, rz = z * h2 - L2 / 2;
+    const double r = sqrt((rx * rx + ry * ry) + rz * rz);
+    double lj;
+    if ((sig == 0.0 || eps == 0.0) && r == 0.0) {
+      lj = 0.0;
+    } else {
+      const double sr = sig / r;
+      const double sr6 = (sr * sr) * (sr * sr) * (sr * sr);
+      const double re = 4 * eps * sr6 * (sr6 - 1);
+      lj = (fabs(re) > eps * 1.0e8) ? eps * 1.0e8 : re;
+    }
+    const double f = erfc(G_COULOMB * r) / r;
+    const double cs = (f > 1.0e3) ? EPSILON0INV * q2 * 1.0e3 : EPSILON0INV * q2 * f;
+    bv[p * g.P + e] = g.beta * (lj + cs);
+  }
+}
+
+// Intramolecular omega_ij(k) = sinc(2 pi r_ij / L0 sqrt(k2)), k2 the
+// integer kz^2 + ky^2 + kx^2 (omega_intra, bgy3d-pure.c:236-267), into
+// w[p * K + e] for the off-diagonal pairs.  rij: [npairs].  Grid: x =
+// chunks of K, y = pair.
+__global__ void __launch_bounds__(NT)
+omega3(GridArgs g, double L0, const double* __restrict__ rij, double* __restrict__ w) {
+  const int p = blockIdx.y;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.K; e += stride) {
+    const int x = (int)(e % g.NH);
+    if (x >= g.NX) continue;
+    const int y = (int)((e / g.NH) % g.N1);
+    const int z = (int)(e / ((int64_t)g.NH * g.N1));
+    const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
+    const int iz = (z <= g.N2 / 2) ? z : z - g.N2;
+    const int k2 = iz * iz + iy * iy + x * x;
+    const double kr = (2.0 * M_PI * rij[p] / L0) * sqrt((double)k2);
+    w[p * g.K + e] = (kr == 0.0) ? 1.0 : sin(kr) / kr;
+  }
+}
+
+// mu densities of the solvent-solvent pairs (print_chempot m x m,
+// hnc3d.c:486-623, 669-687), off-diagonal pairs counted twice: native,
+// HNC, KH, GF and, with gout, g = h + 1 per pair.  uL: the unit long-range
+// Coulomb in real space, cl = -beta q_i q_j uL.  Neumaier sums per thread,
+// part[block * 4 + q].  Grid: x = chunks of P.
+__global__ void __launch_bounds__(NT)
+mu_pairs(GridArgs g, const double* __restrict__ t, const double* __restrict__ bv,
+         const double* __restrict__ uL, double* __restrict__ gout, double* __restrict__ part) {
+  __shared__ double red[4][NT / 64];
+  double sum[4] = {0, 0, 0, 0}, comp[4] = {0, 0, 0, 0};
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.P; e += stride) {
+    double v[4] = {0, 0, 0, 0};
+    for (int i = 0; i < g.msol; ++i)
+      for (int j = 0; j <= i; ++j) {
+        const int p = i * (i + 1) / 2 + j;
+        const int64_t idx = p * g.P + e;
+        const double ti = t[idx], bvi = bv[idx];
+        const double c = closure_c<double>(g.closure, g.pse_order, bvi, ti);
+        const double h = c + ti;
+        const double xx = -bvi + ti;
+        const double cl = -g.beta * g.q[i] * g.q[j] * uL[e];
+        const double mult = (i == j) ? 1.0 : 2.0;
+        v[0] += mult * form_mu(g.closure, g.pse_order, xx, h, c, cl);
+        v[1] += mult * form_mu(0, 0, xx, h, c, cl);
+        v[2] += mult * form_mu(1, 1, xx, h, c, cl);
+        v[3] += mult * form_mu(2, 0, xx, h, c, cl);
+        if (gout) gout[idx] = h + 1.0;
+      }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double s = sum[q] + v[q];
+      comp[q] += (fabs(sum[q]) >= fabs(v[q])) ? (sum[q] - s) + v[q] : (v[q] - s) + sum[q];
+      sum[q] = s;
+    }
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double s = warp_sum(sum[q] + comp[q]);
+    if (lane == 0) red[q][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = red[threadIdx.x][0];
+    for (int i = 1; i < NT / 64; ++i) s += red[threadIdx.x][i];
+    part[(int64_t)blockIdx.x * 4 + threadIdx.x] = s;
+  }
+}
+
+// chi - 1 of the solvent at the corner from the spectrum W_p = FFT(rho h_p)
+// (hnc3d.c:1006-1052): h^3 W s + (omega_p - 1) with omega_ii = 1; out is
+// complex [npairs][N2][N1][NX] without the row padding.  Grid: x = chunks
+// of K, y = pair.
+__global__ void __launch_bounds__(NT)
+chi_from_h(GridArgs g, const cpx<double>* __restrict__ W, const double* __restrict__ omega,
+           double* __restrict__ out) {
+  const int p = blockIdx.y;
+  int i = 0;
+  while ((i + 1) * (i + 2) / 2 <= p) ++i;
+  const int j = p - i * (i + 1) / 2;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  const int64_t Kv = (int64_t)g.NX * g.N1 * g.N2;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.K; e += stride) {
+    const int x = (int)(e % g.NH);
+    if (x >= g.NX) continue;
+    const int y = (int)((e / g.NH) % g.N1);
+    const int z = (int)(e / ((int64_t)g.NH * g.N1));
+    const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
+    const int iz = (z <= g.N2 / 2) ? z : z - g.N2;
+    const double sgn = ((x + iy + iz) & 1) ? -1.0 : 1.0;
+    const cpx<double> wv = W[p * g.K + e];
+    double re = wv.re * g.h3 * sgn, im = wv.im * g.h3 * sgn;
+    if (i != j) re += omega[p * g.K + e];
+    const int64_t o = p * Kv + ((int64_t)z * g.N1 + y) * g.NX + x;
+    out[2 * o] = re;
+    out[2 * o + 1] = im;
+  }
+}
+
+// FP64 -> FP32 copy.
+__global__ void __launch_bounds__(NT)
+to_f32(float* __restrict__ dst, const double* __restrict__ src, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride)
+    dst[e] = (float)src[e];
 }
 
 }  // namespace rism

@@ -52,7 +52,7 @@ struct GridArgs {
   int NX;              // N0/2 + 1: kx points of the half-complex spectrum
   int NH;              // row stride of spectral arrays: NX rounded up to 16,
                        // so that kx tiles start on 128-byte lines
-  int m;               // solvent sites
+  int m;               // solvent sites (pairs in the solvent-solvent mode)
   int nrad;            // 1D table length
   int64_t P, K;        // N0 N1 N2, NH N1 N2 (K counts the padding)
   double dkx, dky, dkz;  // 2 pi / L
@@ -67,6 +67,12 @@ struct GridArgs {
   int pse_order;       // n of PSE-n, 1 for KH
   int zorder;          // zstar grid order (see zstar)
   double q[MAX_SITES]; // solvent charges
+  // zstar variants (0 solute residual): 1 solvent-solvent OZ on site-site
+  // pairs (msol sites, pair p = i(i+1)/2 + j), 2 Poisson potential of a
+  // charge density, 3 the unit long-range Coulomb u_L(k) s(k) / L^3
+  int special;
+  int msol;            // solvent sites of the solvent-solvent mode
+  double rho;          // solvent number density
 };
 
 // Solute sites for the setup and mu kernels, per solute.

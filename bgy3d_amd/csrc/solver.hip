@@ -1054,7 +1054,7 @@ int validate(const Inputs& in) {
   }
   const double dk = M_PI / pd->rmax;
   const int imax = (int)(std::sqrt(k2) / dk - 0.5 + 0.5);
-  if (imax > pd->nrad - 1) return RISM3D_BAD_ARG;
+  if (!in.opt.chi3d && imax > pd->nrad - 1) return RISM3D_BAD_ARG;
   if (in.opt.diis_depth < 1 || in.opt.diis_depth > MAX_DIIS) return RISM3D_BAD_ARG;
   if (in.opt.solver < RISM3D_SOLVER_MDIIS || in.opt.solver > RISM3D_SOLVER_TRIAL)
     return RISM3D_BAD_ARG;
@@ -1255,6 +1255,192 @@ void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& 
 
 std::mutex g_mutex;  // one solve per process at a time (shared streams)
 
+// The 3D solvent-solvent solve (hnc3d_solvent_solve, hnc3d.c:804-1073) on
+// the residual kernels with the m(m+1)/2 site-site pairs as the "sites":
+// per pair the closure of beta v_short and the x/y FFTs, the per-k OZ
+// solve in zstar (special 1), MDIIS on the pairs' t.  Post: mu by the m x m
+// functional with the FFT representation of the long-range Coulomb
+// (special 3), g, and chi - 1 at the corner (mode 10 + forward passes).
+int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
+                const rism3d_options& opt, const double* t_init, double* g_out,
+                double* chi_out, double* t_out, rism3d_result* res) {
+  if (!pd || !solvent || !res || m < 1 || m > 3) return RISM3D_BAD_ARG;
+  for (int d = 0; d < 3; ++d) {
+    if (!supported_n(pd->N[d])) return RISM3D_UNSUPPORTED;
+    if (!(pd->L[d] > 0)) return RISM3D_BAD_ARG;
+  }
+  if (pd->L[0] != pd->L[1] || pd->L[0] != pd->L[2]) return RISM3D_UNSUPPORTED;  // omega_intra
+  if (!(pd->beta > 0) || !(pd->rho > 0)) return RISM3D_BAD_ARG;
+  if (pd->closure < 0 || pd->closure > RISM3D_CLOSURE_PSE7) return RISM3D_BAD_ARG;
+  if (opt.precision != RISM3D_FP64) return RISM3D_UNSUPPORTED;
+  if (opt.solver != RISM3D_SOLVER_MDIIS && opt.solver != RISM3D_SOLVER_PICARD)
+    return RISM3D_UNSUPPORTED;
+  if (opt.diis_depth < 1 || opt.diis_depth > MAX_DIIS) return RISM3D_BAD_ARG;
+  const int np = m * (m + 1) / 2;
+  if ((size_t)np * zstar_lines<double>(pd->N[2], np) * lstride_for(pd->N[2]) * 16 > 64 * 1024)
+    return RISM3D_UNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt.device) return RISM3D_NO_DEVICE;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  HIPCHK(hipSetDevice(opt.device));
+  Problem pb;
+  pb.B = 1;
+  pb.D = opt.solver == RISM3D_SOLVER_MDIIS ? opt.diis_depth : 1;
+  pb.mixed = 0;
+  for (int d = 0; d < 3; ++d) { pb.N[d] = pd->N[d]; pb.L[d] = pd->L[d]; pb.h[d] = pd->L[d] / pd->N[d]; }
+  GridArgs& g = pb.g;
+  std::memset(&g, 0, sizeof g);
+  g.N0 = pd->N[0]; g.N1 = pd->N[1]; g.N2 = pd->N[2];
+  g.NX = pd->N[0] / 2 + 1;
+  g.NH = (g.NX + 15) / 16 * 16;
+  g.m = np;
+  g.msol = m;
+  g.P = (int64_t)g.N0 * g.N1 * g.N2;
+  g.K = (int64_t)g.NH * g.N1 * g.N2;
+  g.dkx = 2 * M_PI / pd->L[0]; g.dky = 2 * M_PI / pd->L[1]; g.dkz = 2 * M_PI / pd->L[2];
+  const double h3 = pb.h[0] * pb.h[1] * pb.h[2];
+  const double L3 = pd->L[0] * pd->L[1] * pd->L[2];
+  g.h3 = h3;
+  g.inv_L3 = 1.0 / L3;
+  g.a_scale = h3 / L3;
+  g.beta = pd->beta;
+  g.rho = pd->rho;
+  g.renorm = 0;
+  g.zorder = env_int("RISM3D_ZSTAR_ORDER", 1) ? 1 : 0;
+  g.closure = pd->closure;
+  g.pse_order = pd->closure == RISM3D_CLOSURE_KH ? 1 : (pd->closure >= RISM3D_CLOSURE_PSE0 ? pd->closure - RISM3D_CLOSURE_PSE0 : 0);
+  for (int i = 0; i < m; ++i) g.q[i] = solvent[i].charge;
+  pb.mP = (int64_t)np * g.P;
+  HIPCHK(hipStreamCreateWithFlags(&pb.s, hipStreamNonBlocking));
+  struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{pb.s};
+  make_twiddles(pb.tw, pb.N, pb.s);
+
+  // omega_ij(k) in the chi3 slots of the pairs
+  pb.chi3 = DevBuf<double>((size_t)np * g.K);
+  {
+    std::vector<double> rij(np, 0.0);
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < i; ++j) {
+        double d2 = 0.0;
+        for (int d = 0; d < 3; ++d) d2 += (solvent[i].x[d] - solvent[j].x[d]) * (solvent[i].x[d] - solvent[j].x[d]);
+        rij[i * (i + 1) / 2 + j] = std::sqrt(d2);
+      }
+    DevBuf<double> d_rij(np);
+    HIPCHK(hipMemcpy(d_rij.p, rij.data(), np * 8, hipMemcpyHostToDevice));
+    const int ck = (int)std::min<int64_t>(4096, (g.K + NT - 1) / NT);
+    hipLaunchKernelGGL(omega3, dim3(ck, np), dim3(NT), 0, pb.s, g, pd->L[0], d_rij.p, pb.chi3.p);
+    HIPCHK(hipStreamSynchronize(pb.s));
+  }
+  pb.st.resize(1);
+  pb.nparts = resid_grid_x(g) * np;
+  auto& s0 = pb.st[0];
+  s0.U = DevBuf<double>((size_t)pb.D * pb.mP);
+  s0.R = DevBuf<double>((size_t)pb.D * pb.mP);
+  s0.t64 = DevBuf<double>(pb.mP);
+  s0.bv64 = DevBuf<double>(pb.mP);
+  s0.W = DevBuf<cpx<double>>((size_t)np * g.K);
+  s0.part = DevBuf<double>((size_t)pb.nparts * NPART);
+  pb.d_sd = DevBuf<SoluteDesc>(1);
+  pb.d_out = DevBuf<double>(NPART);
+  if (t_init)
+    HIPCHK(hipMemcpyAsync(s0.t64.p, t_init, pb.mP * 8, hipMemcpyHostToDevice, pb.s));
+  else
+    HIPCHK(hipMemsetAsync(s0.t64.p, 0, pb.mP * 8, pb.s));
+  {
+    std::vector<double> sse(3 * m);
+    for (int i = 0; i < m; ++i) {
+      sse[3 * i] = solvent[i].sigma;
+      sse[3 * i + 1] = solvent[i].epsilon;
+      sse[3 * i + 2] = solvent[i].charge;
+    }
+    DevBuf<double> d_sse(sse.size());
+    HIPCHK(hipMemcpy(d_sse.p, sse.data(), sse.size() * 8, hipMemcpyHostToDevice));
+    const int cp = (int)std::min<int64_t>(4096, (g.P + NT - 1) / NT);
+    hipLaunchKernelGGL(pair_field, dim3(cp, np), dim3(NT), 0, pb.s, g, d_sse.p, pd->L[0],
+                       pd->L[1], pd->L[2], pb.h[0], pb.h[1], pb.h[2], s0.bv64.p);
+    HIPCHK(hipStreamSynchronize(pb.s));
+    HIPCHK(hipGetLastError());
+  }
+
+  Config cf;
+  cf.solver = opt.solver;
+  cf.precision = RISM3D_FP64;
+  cf.polish = 0;
+  cf.depth = pb.D;
+  cf.verbose = opt.verbose;
+  cf.max_iter = pd->max_iter > 0 ? pd->max_iter : 1000;
+  cf.lam = opt.solver == RISM3D_SOLVER_PICARD ? pd->lambda : opt.diis_step;
+  cf.tol = pd->norm_tol;
+  cf.mixed_tol = opt.mixed_tol;
+  pb.g.special = 1;
+  iterate_batch(pb, cf);
+  res->status = s0.status;
+  res->iterations = s0.iters;
+  res->iterations_fp32 = 0;
+  res->residual = s0.resid;
+
+  // u_L(r) = IFFT(U(k) s(k)) / L^3 into R slot 0 (one copy per pair)
+  const int64_t P = g.P;
+  std::vector<SoluteDesc> sd(1);
+  std::memset(sd.data(), 0, sizeof(SoluteDesc));
+  sd[0].active = 1;
+  sd[0].t = s0.t64.p;
+  sd[0].bv = s0.bv64.p;
+  sd[0].W = s0.W.p;
+  sd[0].part = s0.part.p;
+  sd[0].r = s0.R.p;
+  sd[0].mode = 2;
+  pb.g.special = 3;
+  evaluate(pb, false, sd);
+  pb.g.special = 0;
+  // mu and g
+  {
+    const int cp = (int)std::min<int64_t>(2048, (P + NT - 1) / NT);
+    DevBuf<double> part((size_t)cp * 4);
+    DevBuf<double> gbuf(g_out ? pb.mP : 0);
+    hipLaunchKernelGGL(mu_pairs, dim3(cp), dim3(NT), 0, pb.s, g, s0.t64.p, s0.bv64.p,
+                       s0.R.p, g_out ? gbuf.p : (double*)nullptr, part.p);
+    std::vector<double> hp((size_t)cp * 4);
+    HIPCHK(hipMemcpyAsync(hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost, pb.s));
+    if (g_out) HIPCHK(hipMemcpyAsync(g_out, gbuf.p, pb.mP * 8, hipMemcpyDeviceToHost, pb.s));
+    if (t_out) HIPCHK(hipMemcpyAsync(t_out, s0.t64.p, pb.mP * 8, hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+    double acc[4] = {0, 0, 0, 0};
+    for (int c = 0; c < cp; ++c)
+      for (int q = 0; q < 4; ++q) acc[q] += hp[(size_t)c * 4 + q];
+    const double f = pd->rho * h3 / pd->beta;
+    res->mu = f * acc[0];
+    res->mu_hnc = f * acc[1];
+    res->mu_kh = f * acc[2];
+    res->mu_gf = f * acc[3];
+    res->pmv_factor = NAN;
+  }
+  // chi - 1 at the corner: spectrum of rho h, forward passes only
+  if (chi_out) {
+    sd[0].mode = 10;
+    HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), sizeof(SoluteDesc), hipMemcpyHostToDevice, pb.s));
+    using T = double;
+    constexpr int LN = y_lines<T>();
+    const dim3 gx(resid_grid_x(g), g.m, 1);
+    const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, 1);
+    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
+    launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
+    sd[0].mode = 3;
+    HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), sizeof(SoluteDesc), hipMemcpyHostToDevice, pb.s));
+    launch_zstar<T>(g.N2, g.NX, g.N1, 1, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p,
+                    (const double*)nullptr);
+    const int64_t Kv = (int64_t)g.NX * g.N1 * g.N2;
+    DevBuf<double> cbuf((size_t)np * Kv * 2);
+    const int ck = (int)std::min<int64_t>(4096, (g.K + NT - 1) / NT);
+    hipLaunchKernelGGL(chi_from_h, dim3(ck, np), dim3(NT), 0, pb.s, g,
+                       (const cpx<double>*)s0.W.p, pb.chi3.p, cbuf.p);
+    HIPCHK(hipMemcpyAsync(chi_out, cbuf.p, cbuf.n * 8, hipMemcpyDeviceToHost, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+  }
+  HIPCHK(hipGetLastError());
+  return res->status;
+}
+
 int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
         std::vector<double>* t_final) {
   int v = validate(in);
@@ -1293,7 +1479,7 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   g.a_scale = h3 / L3;
   g.tau_scale = pd.beta * EPSILON0INV / L3;
   g.beta = pd.beta;
-  g.renorm = in.opt.renorm ? 1 : 0;
+  g.renorm = (in.opt.renorm && !in.opt.chi3d) ? 1 : 0;
   g.zorder = env_int("RISM3D_ZSTAR_ORDER", 1) ? 1 : 0;
   g.closure = pd.closure;
   g.pse_order = pd.closure == RISM3D_CLOSURE_KH ? 1 : (pd.closure >= RISM3D_CLOSURE_PSE0 ? pd.closure - RISM3D_CLOSURE_PSE0 : 0);
@@ -1308,6 +1494,22 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   // (solvent_kernel_rism, hnc3d.c:1499-1566; rism_solvent_renorm,
   // rism.f90:141-208).  chi_k is C layout [m][m][nrad].
   const int m = in.m, nrad = pd.nrad;
+  if (in.opt.chi3d) {
+    // --solvent-3d: chi - 1 of a 3D solvent solve on this grid
+    // (solvent_kernel_file, hnc3d.c:1434-1437, 1572-1590), no tau
+    const size_t np = (size_t)m * (m + 1) / 2;
+    pb.chi3 = DevBuf<double>(np * g.K);
+    pb.chi3f = DevBuf<float>(np * g.K);
+    pb.tau3 = DevBuf<double>((size_t)m * g.K);
+    pb.tau3f = DevBuf<float>((size_t)m * g.K);
+    HIPCHK(hipMemcpy2DAsync(pb.chi3.p, g.NH * 8, in.opt.chi3d, g.NX * 8, g.NX * 8,
+                            np * g.N1 * g.N2, hipMemcpyHostToDevice, pb.s));
+    const int ck = (int)std::min<int64_t>(4096, (np * g.K + NT - 1) / NT);
+    hipLaunchKernelGGL(to_f32, dim3(ck), dim3(NT), 0, pb.s, pb.chi3f.p, pb.chi3.p,
+                       (int64_t)(np * g.K));
+    HIPCHK(hipMemsetAsync(pb.tau3.p, 0, (size_t)m * g.K * 8, pb.s));
+    HIPCHK(hipMemsetAsync(pb.tau3f.p, 0, (size_t)m * g.K * 4, pb.s));
+  } else {
   std::vector<double> chi_tab((size_t)m * (m + 1) / 2 * nrad), tau_tab((size_t)m * nrad);
   for (int i = 0; i < m; ++i)
     for (int j = 0; j <= i; ++j)
@@ -1335,6 +1537,7 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     const int ck = (int)std::min<int64_t>(4096, (g.K + NT - 1) / NT);
     hipLaunchKernelGGL(ktab3, dim3(ck), dim3(NT), 0, pb.s, g, pb.chi_tab.p, pb.tau_tab.p,
                        pb.chi3.p, pb.chi3f.p, pb.tau3.p, pb.tau3f.p);
+  }
   }
 
   // per-solute buffers
@@ -1612,7 +1815,7 @@ static int solve_impl(const rism3d_problem* pd, int m, const rism3d_site* solven
   if (opt) in.opt = *opt; else rism3d_options_default(&in.opt);
   std::vector<double> chi_own;
   rism3d_problem pd_up;
-  if (!chi_k) {
+  if (!chi_k && !in.opt.chi3d) {
     // chi from 1D DRISM on the upscaled radial grid (hnc3d.c:772-783,
     // 1471-1495): rmax = 4 max(L)/2, nrad = 16 max(N).
     pd_up = *pd;
@@ -1725,6 +1928,22 @@ int rism3d_jacobian(const rism3d_problem* pd, int m, const rism3d_site* solvent,
     return rism::run(in, &res, nullptr, nullptr);
   } catch (const rism::HipFailure& e) {
     return (e.err == hipErrorOutOfMemory) ? RISM3D_OOM : RISM3D_NO_DEVICE;
+  }
+}
+
+int rism3d_solvent_solve(const rism3d_problem* pd, int m, const rism3d_site* solvent,
+                         const rism3d_options* opt, const double* t_init, double* g_out,
+                         double* chi_out, double* t_out, rism3d_result* res) {
+  rism3d_options o;
+  if (opt) o = *opt; else rism3d_options_default(&o);
+  try {
+    const int st = rism::solvent_run(pd, m, solvent, o, t_init, g_out, chi_out, t_out, res);
+    if (res && st != RISM3D_OK && st != RISM3D_NONCONVERGED && st != RISM3D_NAN) res->status = st;
+    return st;
+  } catch (const rism::HipFailure& e) {
+    const int st = (e.err == hipErrorOutOfMemory) ? RISM3D_OOM : RISM3D_NO_DEVICE;
+    if (res) res->status = st;
+    return st;
   }
 }
 

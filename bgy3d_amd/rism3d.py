@@ -64,7 +64,8 @@ class Options(ctypes.Structure):
                 ("mixed_tol", ctypes.c_double), ("diis_depth", ctypes.c_int),
                 ("diis_step", ctypes.c_double), ("verbose", ctypes.c_int),
                 ("device", ctypes.c_int), ("dielectric", ctypes.c_double),
-                ("continuation", ctypes.c_int)]
+                ("continuation", ctypes.c_int),
+                ("chi3d", ctypes.POINTER(ctypes.c_double))]
 
 
 class Result(ctypes.Structure):
@@ -82,7 +83,8 @@ class Result(ctypes.Structure):
 
 
 EXPORTS = ("rism3d_options_default", "rism3d_solute_solve", "rism3d_batch_solve",
-           "rism3d_residual", "rism3d_jacobian", "rism3d_restart_destroy",
+           "rism3d_residual", "rism3d_jacobian", "rism3d_solvent_solve",
+           "rism3d_restart_destroy",
            "rism3d_solvent_chi", "rism3d_version", "rism3d_profile_start",
            "rism3d_profile_read")
 
@@ -116,6 +118,9 @@ def load():
         lib.rism3d_jacobian.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_int,
                                         P(Site), dp, P(Options), dp, dp, ctypes.c_int, dp]
         lib.rism3d_jacobian.restype = ctypes.c_int
+        lib.rism3d_solvent_solve.argtypes = [P(Problem), ctypes.c_int, P(Site), P(Options), dp,
+                                             dp, dp, dp, P(Result)]
+        lib.rism3d_solvent_solve.restype = ctypes.c_int
         lib.rism3d_restart_destroy.argtypes = [ctypes.c_void_p]
         lib.rism3d_restart_destroy.restype = None
         lib.rism3d_solvent_chi.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_double, dp]
@@ -213,7 +218,7 @@ def _dptr(a):
 
 
 def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=False,
-                     derivatives=False, adjoint=False, response=False):
+                     derivatives=False, adjoint=False, response=False, chi3d=None):
     """Solve the 3D-RISM solute problem on the GPU.
 
     Returns a dict like the reference alist: 'free-energy',
@@ -226,7 +231,9 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
     response=True (or the 'response setting, hnc3d.c:1784) adds
     'free-energy-response' (n, 3), its linear-response gradient;
     adjoint=True adds 'free-energy-dsite' (n, 6), exact
-    d mu / d(x, y, z, q, sigma, epsilon) by the adjoint."""
+    d mu / d(x, y, z, q, sigma, epsilon) by the adjoint.
+    chi3d: the --solvent-3d susceptibility of hnc3d_run_solvent ('chi-3d',
+    complex or its real part), in place of chi (no tau renormalisation)."""
     lib = load()
     pd = problem_data(settings)
     opt = make_options(settings)
@@ -254,6 +261,17 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
         dsite = np.full((len(su), 6), np.nan)
         res.dmu_dsite = _dptr(dsite)
     chi_p = None
+    if chi3d is not None:
+        c3 = np.asarray(chi3d)
+        if np.iscomplexobj(c3):
+            c3 = c3.real
+        elif c3.ndim == 5:
+            c3 = c3[..., 0]
+        c3 = np.ascontiguousarray(c3, dtype=np.float64)
+        want = (m * (m + 1) // 2, pd.N[2], pd.N[1], pd.N[0] // 2 + 1)
+        if c3.shape != want:
+            raise ValueError(f"chi3d must be {want}, got {c3.shape}")
+        opt.chi3d = _dptr(c3)
     if chi is not None:
         chi = np.ascontiguousarray(chi, dtype=np.float64)
         if chi.shape != (m, m, pd.nrad):
@@ -286,6 +304,49 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
         out["adjoint-iterations"] = res.adjoint_iterations
     if tok is not None:
         out["RESTART"] = tok.value
+    return out
+
+
+def hnc3d_run_solvent(solvent, settings, t_init=None, save=None):
+    """The 3D solvent-solvent solve on the GPU (hnc3d_solvent_solve,
+    hnc3d.c:804-1073).  Returns 'free-energy' (native closure) and
+    'free-energy-hnc'/'-kh'/'-gf', 'GVV' (pairs p = i(i+1)/2 + j, j <= i,
+    [np][N2][N1][N0]), 'chi-3d' (complex chi - 1 at the corner,
+    [np][N2][N1][N0/2+1], the input of hnc3d_run_solute(chi3d=)), 't',
+    'iterations', 'residual', 'status'.  save: a directory to write the
+    reference's files g%d%d.bin and x%d%d-fft.bin (PETSc binary)."""
+    lib = load()
+    pd = problem_data(settings)
+    opt = make_options(settings)
+    sv = make_sites(solvent)
+    m = len(sv)
+    npair = m * (m + 1) // 2
+    shape = (npair, pd.N[2], pd.N[1], pd.N[0])
+    g = np.empty(shape)
+    t = np.empty(shape)
+    chi = np.empty((npair, pd.N[2], pd.N[1], pd.N[0] // 2 + 1, 2))
+    ti = None
+    if t_init is not None:
+        ti = np.ascontiguousarray(t_init, dtype=np.float64)
+        if ti.shape != shape:
+            raise ValueError(f"t_init must be {shape}, got {ti.shape}")
+    res = Result()
+    st = lib.rism3d_solvent_solve(ctypes.byref(pd), m, sv, ctypes.byref(opt),
+                                  _dptr(ti) if ti is not None else None, _dptr(g), _dptr(chi),
+                                  _dptr(t), ctypes.byref(res))
+    if st not in (OK, NONCONVERGED):
+        raise RismError(st, "rism3d_solvent_solve")
+    out = {"free-energy": res.mu, "free-energy-hnc": res.mu_hnc, "free-energy-kh": res.mu_kh,
+           "free-energy-gf": res.mu_gf, "GVV": g, "chi-3d": chi[..., 0] + 1j * chi[..., 1],
+           "t": t, "iterations": res.iterations, "residual": res.residual,
+           "status": STATUS_NAMES[st]}
+    if save is not None:
+        from . import petsc_io
+        pairs = [(i, j) for i in range(m) for j in range(i + 1)]
+        for p, (i, j) in enumerate(pairs):
+            for a, b in ((i, j), (j, i)):
+                petsc_io.write_vec(os.path.join(save, f"g{a}{b}.bin"), g[p])
+                petsc_io.write_vec(os.path.join(save, f"x{a}{b}-fft.bin"), chi[p])
     return out
 
 

@@ -132,6 +132,13 @@ typedef struct rism3d_options {
                           loosely converged (default 1; 0: report NAN).
                           The reference has no such fallback; the fixed
                           point is unchanged                               */
+  const double *chi3d; /* NULL, or the --solvent-3d susceptibility: chi - 1
+                          of a 3D solvent solve on this grid
+                          (rism3d_solvent_solve's chi_out, real part),
+                          double[m(m+1)/2][N2][N1][N0/2+1] by pair
+                          i(i+1)/2 + j, j <= i (solvent_kernel_file,
+                          hnc3d.c:1434-1437, 1572-1590).  Replaces chi_k;
+                          no tau renormalisation, as in the reference      */
 } rism3d_options;
 
 typedef struct rism3d_result {
@@ -226,6 +233,27 @@ int rism3d_jacobian(const rism3d_problem *pd,
                     const rism3d_options *opt,
                     const double *t, const double *dt, int flag,
                     double *jdt);
+
+/* The 3D solvent-solvent solve, replacing hnc3d_solvent_solve
+   (hnc3d.c:804-1073): site-site pair correlations on the 3D grid, centred
+   at the box centre, from the OZ equation with the intramolecular
+   omega(k); m <= 3 solvent sites, cubic box, FP64, MDIIS or Picard.
+   Pairs are indexed p = i(i+1)/2 + j, j <= i.
+   t_init: NULL (start from t = 0) or double[m(m+1)/2][N2][N1][N0];
+   g_out:  NULL or double[m(m+1)/2][N2][N1][N0], g_ij = 1 + h_ij (the
+           reference's g%d%d.bin);
+   chi_out: NULL or double[m(m+1)/2][N2][N1][N0/2+1][2], complex
+           chi - 1 at the grid corner (the reference's x%d%d-fft.bin; its
+           real part is what rism3d_options.chi3d takes);
+   t_out:  NULL or double[m(m+1)/2][N2][N1][N0], the converged t_ij;
+   res:    mu (native closure), mu_hnc/_kh/_gf (print_chempot of the m x m
+           pairs, hnc3d.c:669-687), residual, iterations, status. */
+int rism3d_solvent_solve(const rism3d_problem *pd,
+                         int m, const rism3d_site *solvent,
+                         const rism3d_options *opt,
+                         const double *t_init,
+                         double *g_out, double *chi_out, double *t_out,
+                         rism3d_result *res);
 
 /* Replaces bgy3d_restart_destroy (bgy3d-impure.c:593-600). */
 void rism3d_restart_destroy(rism3d_restart *restart);

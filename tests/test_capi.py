@@ -122,6 +122,28 @@ def test_validation_before_device():
     assert e.value.status == R.UNSUPPORTED
 
 
+def test_solvent_solve_validation():
+    """rism3d_solvent_solve: m <= 3 sites (the per-k OZ is solved by
+    elimination in registers), cubic boxes only (omega_ij(k) takes L[0] for
+    every axis, as bgy3d-pure.c:236-267), FP64 MDIIS or Picard."""
+    R = _R()
+    base = {"L": 10, "N": 32}
+    with pytest.raises(R.RismError) as e:
+        R.hnc3d_run_solvent("methanol", base)          # 6 sites
+    assert e.value.status == R.BAD_ARG
+    with pytest.raises(R.RismError) as e:
+        R.hnc3d_run_solvent("water", dict(base, precision="mixed"))
+    assert e.value.status == R.UNSUPPORTED
+    with pytest.raises(R.RismError) as e:
+        R.hnc3d_run_solvent("water", dict(base, **{"snes-solver": "newton"}))
+    assert e.value.status == R.UNSUPPORTED
+    with pytest.raises(R.RismError) as e:
+        R.hnc3d_run_solvent("water", dict(base, N=100))
+    assert e.value.status == R.UNSUPPORTED
+    with pytest.raises(ValueError):
+        R.hnc3d_run_solvent("water", base, t_init=np.zeros((3, 32, 32, 32)))
+
+
 @pytest.mark.skipif(_gpu_present(), reason="checks the no-GPU error path")
 def test_no_device_fails_loudly():
     R = _R()
@@ -129,3 +151,16 @@ def test_no_device_fails_loudly():
     with pytest.raises(R.RismError) as e:
         R.hnc3d_run_solute(W, W, {"L": 10, "N": 32, "rmax": 40.0, "nrad": 1536}, chi=_chi())
     assert e.value.status == R.NO_DEVICE
+
+
+def test_petsc_vec_format(tmp_path):
+    """bgy3d_vec_save layout: int32 class id 1211214, int32 length, then
+    big-endian doubles in natural order."""
+    from bgy3d_amd import petsc_io
+    a = np.arange(24, dtype=np.float64).reshape(2, 3, 4) * 0.5
+    path = tmp_path / "g0.bin"
+    petsc_io.write_vec(path, a)
+    raw = open(path, "rb").read()
+    assert raw[:4] == (1211214).to_bytes(4, "big") and raw[4:8] == (24).to_bytes(4, "big")
+    assert np.frombuffer(raw[8:], dtype=">f8")[5] == 2.5
+    assert np.array_equal(petsc_io.read_vec(path, a.shape), a)

@@ -1,0 +1,150 @@
+"""The --solvent-3d path on the GPU: rism3d_solvent_solve (the 3D
+solvent-solvent solve, hnc3d_solvent_solve) and rism3d_solute_solve with
+its chi (options.chi3d).
+
+* 32^3: GPU vs the oracle (oracle/solvent3d.py) on the same converged
+  problem: mu to 1e-8, g and chi - 1 to 1e-8.
+* 96^3, the reference's regression settings (test/Makefile HNC-FLAGS,
+  norm-tol 1e-12: the solute's mu moves by 2e-7 between a solvent
+  converged to 1e-9 and to 1e-11, enough to flip the 6th decimal):
+  test/out/water.hnc3d.g2 (mu by HNC/KH/GF to 6 decimals, moments of
+  1 - g_ij to 4 digits) and test/out/water.hnc3d.g1 for the solute "water"
+  on the GPU's own chi (mu, moments of 1 - g_i).
+* PETSc binary files written by the Python mirror read back bit for bit.
+"""
+import numpy as np
+import pytest
+
+from oracle import closures as C
+from oracle import rism1d as R1
+from oracle import rism3d as O
+from oracle import solvent3d as S3
+
+pytestmark = pytest.mark.gpu
+BETA, RHO = 1.6889, 0.033427745
+
+
+def _settings(N, L, **kw):
+    return dict({"N": N, "L": L, "beta": BETA, "rho": RHO, "closure": "KH", "rmax": 40.0,
+                 "nrad": 16 * N, "max-iter": 2000}, **kw)
+
+
+def _guess(pd, s):
+    one = R1.rism_vv(C.KH, pd.nrad, pd.rmax, BETA, RHO, s.solvent)
+    return s.initial_guess(one["t"], pd.rmax)
+
+
+def test_solvent_vs_oracle_32():
+    print("case solvent3d n32 vs oracle", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd.solutes import find_molecule
+    W = find_molecule("water")
+    pd = O.Problem.from_cli(8.0, 32, BETA, RHO, closure=C.KH, rmax=40.0, nrad=512)
+    ref = S3.Solvent3D(pd, W)
+    t0 = _guess(pd, ref)
+    t, _ = ref.solve(t0=t0, atol=1e-11, max_iter=2000)
+    obs = ref.observables(t)
+    out = R.hnc3d_run_solvent("water", _settings(32, 8.0, **{"norm-tol": 1e-11}), t_init=t0)
+    print(f"  {out['status']} iters {out['iterations']} mu {out['free-energy']:.10f} "
+          f"oracle {obs['mu_kh']:.10f}", flush=True)
+    assert out["status"] == "OK"
+    assert abs(out["free-energy"] - obs["mu_kh"]) < 1e-8
+    assert abs(out["free-energy-hnc"] - obs["mu_hnc"]) < 1e-8
+    assert abs(out["free-energy-gf"] - obs["mu_gf"]) < 1e-8
+    dg = np.max(np.abs(out["GVV"] - obs["g"]))
+    dchi = max(np.max(np.abs(out["chi-3d"][p] - obs["chi"][ij]))
+               for p, ij in enumerate(ref.pairs))
+    print(f"  max|dg| {dg:.2e} max|dchi| {dchi:.2e}", flush=True)
+    assert dg < 1e-8 and dchi < 1e-8
+
+
+def test_solute_on_chi3d_vs_oracle_32():
+    """The solute kernel on the oracle's own 3D chi (same input table):
+    mu and g against oracle/rism3d.py Solute3D(chi3d=)."""
+    print("case solute water on chi3d n32 vs oracle", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd.solutes import find_molecule
+    W = find_molecule("water")
+    pd = O.Problem.from_cli(8.0, 32, BETA, RHO, closure=C.KH, rmax=40.0, nrad=512)
+    ref = S3.Solvent3D(pd, W)
+    t, _ = ref.solve(t0=_guess(pd, ref), atol=1e-11, max_iter=2000)
+    chi = S3.chi_table(ref.observables(t), 3)
+    u = O.Solute3D(pd, W, W, None, chi3d=chi)
+    tu, _ = u.solve(atol=1e-11)
+    o = u.observables(tu)
+    out = R.hnc3d_run_solute("water", "water", _settings(32, 8.0, **{"norm-tol": 1e-11}),
+                             chi3d=chi)
+    print(f"  {out['status']} mu {out['free-energy']:.10f} oracle {o['mu_kh']:.10f}", flush=True)
+    assert out["status"] == "OK"
+    assert abs(out["free-energy"] - o["mu_kh"]) < 1e-8
+    assert abs(out["free-energy-hnc"] - o["mu_hnc"]) < 1e-8
+    assert abs(out["free-energy-gf"] - o["mu_gf"]) < 1e-8
+    assert np.max(np.abs(out["GUV"] - o["g"])) < 1e-8
+
+
+@pytest.fixture(scope="module")
+def solvent96():
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd.solutes import find_molecule
+    W = find_molecule("water")
+    pd = O.Problem.from_cli(10.0, 96, BETA, RHO, closure=C.KH, rmax=40.0, nrad=1536)
+    t0 = _guess(pd, S3.Solvent3D(pd, W))
+    out = R.hnc3d_run_solvent("water", _settings(96, 10.0, **{"norm-tol": 1e-12}), t_init=t0)
+    return pd, out
+
+
+def _f4(x):
+    return float(f"{x:.4f}")
+
+
+def test_water_hnc3d_g2_gpu(solvent96):
+    print("case solvent3d n96 water.hnc3d.g2", flush=True)
+    pd, out = solvent96
+    print(f"  {out['status']} iters {out['iterations']} mu HNC {out['free-energy-hnc']:.6f} "
+          f"KH {out['free-energy-kh']:.6f} GF {out['free-energy-gf']:.6f}", flush=True)
+    assert out["status"] == "OK"
+    assert round(out["free-energy-hnc"], 6) == -4.161039
+    assert round(out["free-energy-kh"], 6) == -5.793348
+    assert round(out["free-energy-gf"], 6) == -9.595354
+    golden = [(27.9513, -0.5919, 1.1791, -0.0155), (27.9513, -0.3764, 9.4259, -0.0035),
+              (27.9513, -0.2406, 11.5147, 0.0143), (27.9513, -0.3764, 9.4259, -0.0035),
+              (27.9513, -0.2406, 11.5147, 0.0143), (27.9513, -0.2406, 11.5147, 0.0143)]
+    for p in range(6):  # pairs (0,0) (1,0) (1,1) (2,0) (2,1) (2,2)
+        q, d, Q = S3.moments(pd, 1.0 - out["GVV"][p])
+        m0, m1, diag, off = golden[p]
+        assert _f4(q) == m0
+        assert [_f4(x) for x in d] == [m1] * 3
+        assert [_f4(Q[a][a]) for a in range(3)] == [diag] * 3
+        assert [_f4(Q[a][b]) for a in range(3) for b in range(3) if a != b] == [off] * 6
+
+
+def test_water_hnc3d_g1_gpu(solvent96):
+    print("case solute water on the GPU chi-3d, water.hnc3d.g1", flush=True)
+    from bgy3d_amd import rism3d as R
+    pd, sv = solvent96
+    out = R.hnc3d_run_solute("water", "water", _settings(96, 10.0, **{"norm-tol": 1e-12}),
+                             chi3d=sv["chi-3d"])
+    print(f"  {out['status']} iters {out['iterations']} mu HNC {out['free-energy-hnc']:.6f} "
+          f"KH {out['free-energy-kh']:.6f} GF {out['free-energy-gf']:.6f}", flush=True)
+    assert out["status"] == "OK"
+    assert round(out["free-energy-hnc"], 6) == -2.284209
+    assert round(out["free-energy-kh"], 6) == -3.894980
+    assert round(out["free-energy-gf"], 6) == -5.870259
+    golden = [(30.3041, (-3.4139, -0.4309, -0.8353), (9.4396, 14.2357, 9.4527)),
+              (30.3041, (12.9075, -0.1765, -0.5926), (10.4809, 31.3124, 9.9790)),
+              (30.3041, (12.9075, -0.1765, -0.5926), (10.4809, 31.3124, 9.9790))]
+    for i, (m0, m1, diag) in enumerate(golden):
+        q, d, Q = S3.moments(pd, 1.0 - out["GUV"][i])
+        assert _f4(q) == m0
+        assert tuple(_f4(x) for x in d) == m1
+        assert tuple(_f4(Q[a][a]) for a in range(3)) == diag
+
+
+def test_petsc_files_round_trip(solvent96, tmp_path):
+    from bgy3d_amd import petsc_io
+    pd, out = solvent96
+    chi = out["chi-3d"]
+    arr = np.stack([chi[1].real, chi[1].imag], axis=-1)
+    petsc_io.write_vec(tmp_path / "x10-fft.bin", arr)
+    back = petsc_io.read_vec(tmp_path / "x10-fft.bin", arr.shape)
+    assert np.array_equal(back, arr)
