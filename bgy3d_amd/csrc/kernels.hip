@@ -1305,4 +1305,146 @@ to_f32(float* __restrict__ dst, const double* __restrict__ src, int64_t n) {
     dst[e] = (float)src[e];
 }
 
+// ------------------------------------------------------ info observables
+// Solvent charge density rho_v = sum_i (q_i rho) g_i with g_i = (c_i + t_i)
+// + 1 (bgy3d_solvent_field, bgy3d-potential.c:246-262, on the g that
+// hnc3d.c:2319-2325 passes).  One solute; grid-stride over P.
+__global__ void __launch_bounds__(NT)
+info_rho_v(GridArgs g, const double* __restrict__ t, const double* __restrict__ bv, double rho,
+           double* __restrict__ rhov) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.P; e += stride) {
+    double acc = 0.0;
+    for (int i = 0; i < g.m; ++i) {
+      const int64_t idx = i * g.P + e;
+      const double c = closure_c<double>(g.closure, g.pse_order, bv[idx], t[idx]);
+      acc += (g.q[i] * rho) * ((c + t[idx]) + 1.0);
+    }
+    rhov[e] = acc;
+  }
+}
+
+// The grid sums of print_info (bgy3d-potential.c:341-443), unscaled, with
+// X_a = i_a - N_a / 2 (bgy3d_moments, bgy3d-vec.c:573-593):
+//   [0..9]   rho_v: 1, X, Y, Z, XX, XY, XZ, YY, YZ, ZZ
+//   [10..19] rho_u, the Gaussian cores (G/sqrt(pi))^3 sum_a q_a
+//            exp(-G^2 r_a^2) (bgy3d-solutes.c:555-577), same moments
+//   [20] U_v rho_u   [21] U_u rho_v, U_u = eps0^-1 sum_a q_a G cl0(G r_a)
+// Neumaier sums per thread, part[block * INFO_NSUM + q].  Grid: x = chunks
+// of P.
+__global__ void __launch_bounds__(NT)
+info_sums(GridArgs g, SitesDesc sd, const double* __restrict__ rhov,
+          const double* __restrict__ ve, double L0, double L1, double L2, double h0, double h1,
+          double h2, double* __restrict__ part) {
+  __shared__ double red[INFO_NSUM][NT / 64];
+  double sum[INFO_NSUM], comp[INFO_NSUM];
+#pragma unroll
+  for (int q = 0; q < INFO_NSUM; ++q) { sum[q] = 0.0; comp[q] = 0.0; }
+  const double gn = G_COULOMB / sqrt(M_PI);
+  const double norm = gn * gn * gn;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.P; e += stride) {
+    const int x = (int)(e % g.N0);
+    const int y = (int)((e / g.N0) % g.N1);
+    const int z = (int)(e / ((int64_t)g.N0 * g.N1));
+    const double rx = x * h0 - L0 / 2, ry = y * h1 - L1 / 2, rz = z * h2 - L2 / 2;
+    double ru = 0.0, uc = 0.0;
+    for (int a = 0; a < sd.n; ++a) {
+      const double* s = sd.xyz_s_e_q + 6 * a;
+      const double dx = rx - s[0], dy = ry - s[1], dz = rz - s[2];
+      const double r2 = (dx * dx + dy * dy) + dz * dz;
+      ru += s[5] * exp(-G_COULOMB * G_COULOMB * r2);
+      uc += s[5] * G_COULOMB * cl0(G_COULOMB * sqrt(r2));
+    }
+    ru *= norm;
+    uc *= EPSILON0INV;
+    const double X = x - 0.5 * g.N0, Y = y - 0.5 * g.N1, Z = z - 0.5 * g.N2;
+    const double rv = rhov[e];
+    const double w[10] = {1.0, X, Y, Z, X * X, X * Y, X * Z, Y * Y, Y * Z, Z * Z};
+    double v[INFO_NSUM];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) { v[q] = rv * w[q]; v[10 + q] = ru * w[q]; }
+    v[20] = ve[e] * ru;
+    v[21] = uc * rv;
+#pragma unroll
+    for (int q = 0; q < INFO_NSUM; ++q) {
+      const double s = sum[q] + v[q];
+      comp[q] += (fabs(sum[q]) >= fabs(v[q])) ? (sum[q] - s) + v[q] : (v[q] - s) + sum[q];
+      sum[q] = s;
+    }
+  }
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < INFO_NSUM; ++q) {
+    const double s = warp_sum(sum[q] + comp[q]);
+    if (lane == 0) red[q][wv] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < INFO_NSUM) {
+    double s = red[threadIdx.x][0];
+    for (int i = 1; i < NT / 64; ++i) s += red[threadIdx.x][i];
+    part[(int64_t)blockIdx.x * INFO_NSUM + threadIdx.x] = s;
+  }
+}
+
+// The solvent potential at the solute sites by trigonometric interpolation
+// (bgy3d_pot_interp, bgy3d-potential.c:125-143, and bgy3d_fft_interp,
+// bgy3d-fftw3.c:504-612) straight from Y = FFT(rho_v) (spectrum of site 0
+// in W, padded rows): FFT(U_v) = N0 N1 N2 Y f / L^3 with the Poisson factor
+// f = 4 pi eps0^-1 h^3 / k^2 (0 at k = 0), so
+//   U_v(y) = sum_k 2 Re(Y f / L^3 w_x w_y w_z) / (2 if k_x = 0 or N0/2),
+// w_d = exp(2 pi i K_d y_d / N_d), cos(...) where K_d = -K_d mod N_d, and
+// y_d = (x_d + L_d / 2) / h_d.  xs: [n][3].  part[(a * gridDim.x + block)].
+// Grid: x = chunks of K, y = site.
+__global__ void __launch_bounds__(NT)
+info_interp(GridArgs g, const cpx<double>* __restrict__ W, const double* __restrict__ xs,
+            double L0, double L1, double L2, double h0, double h1, double h2,
+            double* __restrict__ part) {
+  __shared__ double red[NT / 64];
+  const int a = blockIdx.y;
+  const double yx = (xs[3 * a] + L0 / 2) / h0;
+  const double yy = (xs[3 * a + 1] + L1 / 2) / h1;
+  const double yz = (xs[3 * a + 2] + L2 / 2) / h2;
+  double sum = 0.0, comp = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.K; e += stride) {
+    const int x = (int)(e % g.NH);
+    if (x >= g.NX) continue;
+    const int y = (int)((e / g.NH) % g.N1);
+    const int z = (int)(e / ((int64_t)g.NH * g.N1));
+    const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
+    const int iz = (z <= g.N2 / 2) ? z : z - g.N2;
+    const double kx = x * g.dkx, ky = iy * g.dky, kz = iz * g.dkz;
+    const double k2 = (kz * kz + ky * ky) + kx * kx;
+    if (k2 == 0.0) continue;
+    const double f = 4 * M_PI * EPSILON0INV * g.h3 / k2 * g.inv_L3;
+    // the three one-dimensional waves
+    double cx, sx, cy, sy, cz, sz;
+    sincos((2 * M_PI / g.N0) * x * yx, &sx, &cx);
+    sincos((2 * M_PI / g.N1) * iy * yy, &sy, &cy);
+    sincos((2 * M_PI / g.N2) * iz * yz, &sz, &cz);
+    if ((g.N0 - 2 * x) % g.N0 == 0) sx = 0.0;
+    if ((g.N1 - 2 * y) % g.N1 == 0) sy = 0.0;
+    if ((g.N2 - 2 * z) % g.N2 == 0) sz = 0.0;
+    // w = wx wy wz
+    const double ar = cx * cy - sx * sy, ai = cx * sy + sx * cy;
+    const double wr = ar * cz - ai * sz, wi = ar * sz + ai * cz;
+    const cpx<double> Yk = W[e];
+    double v = 2.0 * f * (Yk.re * wr - Yk.im * wi);
+    if ((g.N0 - 2 * x) % g.N0 == 0) v *= 0.5;
+    const double s = sum + v;
+    comp += (fabs(sum) >= fabs(v)) ? (sum - s) + v : (v - s) + sum;
+    sum = s;
+  }
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double s = warp_sum(sum + comp);
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double acc = red[0];
+    for (int i = 1; i < NT / 64; ++i) acc += red[i];
+    part[(int64_t)a * gridDim.x + blockIdx.x] = acc;
+  }
+}
+
 }  // namespace rism

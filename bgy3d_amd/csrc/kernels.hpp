@@ -45,6 +45,7 @@ struct SoluteDesc {
   double shift_w;            //   entries of the previous base moved to this one
 };
 constexpr int NPART = 2 + MAX_DIIS;   // doubles per residual workgroup
+constexpr int INFO_NSUM = 22;         // grid sums of the info observables
 
 // Grid and physics constants shared by a batch.
 struct GridArgs {

@@ -1084,6 +1084,112 @@ int validate(const Inputs& in) {
 // sum_i <A_i | dv_i> with A_i = -beta (D_i + 1) (K lambda)_i + dmu/dv_i,
 // one field reduction.  The adjoint adds Lambda = r2c(lambda) and the
 // tau and u_c terms (grad_tau, grad_field mode 1).
+// info / print_info (bgy3d-potential.c:341-491, cage = false) for solute b
+// after its solve: rho_v, its Poisson potential U_v (a pass of the residual
+// pipeline with one "site", the closure step copying rho_v and the k-space
+// step the Poisson factor), the grid sums and the interpolated U_v at the
+// sites.  W and R slot 0 of the solute are reused as work space.
+void info_observables(Problem& pb, const Inputs& in, int b, const SitesDesc& sites,
+                      rism3d_info* out) {
+  GridArgs& g = pb.g;
+  const rism3d_problem& pd = *in.pd;
+  auto& s = pb.st[b];
+  const int n = in.n_sites[b];
+  const rism3d_site* su = in.solutes[b];
+  const int64_t P = g.P;
+  DevBuf<double> rhov(P), ve(P);
+  const int cp = (int)std::min<int64_t>(2048, (P + NT - 1) / NT);
+  hipLaunchKernelGGL(info_rho_v, dim3(cp), dim3(NT), 0, pb.s, g, s.t64.p, s.bv64.p, pd.rho,
+                     rhov.p);
+  // U_v = IFFT(FFT(rho_v) f) / L^3, f = 4 pi eps0^-1 h^3 / k^2
+  std::vector<SoluteDesc> sd(pb.B);
+  std::memset(sd.data(), 0, sd.size() * sizeof(SoluteDesc));
+  sd[b].active = 1;
+  sd[b].t = rhov.p;
+  sd[b].bv = s.bv64.p;
+  sd[b].W = s.W.p;
+  sd[b].part = s.part.p;
+  sd[b].r = ve.p;
+  sd[b].mode = 2;
+  const int m_save = g.m, special_save = g.special;
+  g.m = 1;
+  g.special = 2;
+  evaluate(pb, false, sd);
+  // Y = FFT(rho_v) alone in W for the interpolation (forward passes)
+  g.special = 0;
+  sd[b].mode = 3;
+  HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), sd.size() * sizeof(SoluteDesc),
+                        hipMemcpyHostToDevice, pb.s));
+  {
+    using T = double;
+    constexpr int LN = y_lines<T>();
+    const dim3 gx(resid_grid_x(g), 1, pb.B);
+    const dim3 gy((g.NX + LN - 1) / LN, g.N2, pb.B);
+    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
+    launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
+    launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p,
+                    pb.tau3.p);
+  }
+  g.m = m_save;
+  g.special = special_save;
+  DevBuf<double> part((size_t)cp * INFO_NSUM);
+  hipLaunchKernelGGL(info_sums, dim3(cp), dim3(NT), 0, pb.s, g, sites, rhov.p, ve.p, pd.L[0],
+                     pd.L[1], pd.L[2], pb.h[0], pb.h[1], pb.h[2], part.p);
+  std::vector<double> xs(3 * (size_t)n);
+  for (int a = 0; a < n; ++a)
+    for (int d = 0; d < 3; ++d) xs[3 * a + d] = su[a].x[d];
+  DevBuf<double> d_xs(xs.size());
+  HIPCHK(hipMemcpyAsync(d_xs.p, xs.data(), xs.size() * 8, hipMemcpyHostToDevice, pb.s));
+  const int ck = (int)std::min<int64_t>(512, (g.K + NT - 1) / NT);
+  DevBuf<double> ipart((size_t)n * ck);
+  hipLaunchKernelGGL(info_interp, dim3(ck, n), dim3(NT), 0, pb.s, g, (const cpx<double>*)s.W.p,
+                     d_xs.p, pd.L[0], pd.L[1], pd.L[2], pb.h[0], pb.h[1], pb.h[2], ipart.p);
+  std::vector<double> hp((size_t)cp * INFO_NSUM), hi((size_t)n * ck);
+  HIPCHK(hipMemcpyAsync(hp.data(), part.p, hp.size() * 8, hipMemcpyDeviceToHost, pb.s));
+  HIPCHK(hipMemcpyAsync(hi.data(), ipart.p, hi.size() * 8, hipMemcpyDeviceToHost, pb.s));
+  if (out->potential)
+    HIPCHK(hipMemcpyAsync(out->potential, ve.p, P * 8, hipMemcpyDeviceToHost, pb.s));
+  HIPCHK(hipStreamSynchronize(pb.s));
+  HIPCHK(hipGetLastError());
+  double acc[INFO_NSUM];
+  for (int q = 0; q < INFO_NSUM; ++q) {
+    double sum = 0.0, comp = 0.0;
+    for (int c = 0; c < cp; ++c) {
+      const double v = hp[(size_t)c * INFO_NSUM + q];
+      const double t = sum + v;
+      comp += (std::fabs(sum) >= std::fabs(v)) ? (sum - t) + v : (v - t) + sum;
+      sum = t;
+    }
+    acc[q] = sum + comp;
+  }
+  const double* h = pb.h;
+  const double dV = h[0] * h[1] * h[2];
+  auto moments = [&](const double* v, double* q, double* d, double (*Q)[3]) {
+    *q = dV * v[0];
+    for (int a = 0; a < 3; ++a) d[a] = dV * h[a] * v[1 + a];
+    const int idx[3][3] = {{4, 5, 6}, {5, 7, 8}, {6, 8, 9}};
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) Q[a][c] = dV * h[a] * h[c] * v[idx[a][c]];
+  };
+  moments(acc, &out->q_v, out->d_v, out->Q_v);
+  moments(acc + 10, &out->q_u, out->d_u, out->Q_u);
+  out->U_v_rho_u = dV * acc[20];
+  out->rho_v_U_u = dV * acc[21];
+  for (int d = 0; d < 3; ++d) {
+    double dn = 0.0;
+    for (int a = 0; a < n; ++a) dn += su[a].charge * su[a].x[d];
+    out->d_n[d] = dn;
+  }
+  double val1 = 0.0;
+  for (int a = 0; a < n; ++a) {
+    double sum = 0.0;
+    for (int c = 0; c < ck; ++c) sum += hi[(size_t)a * ck + c];
+    if (out->v_sites) out->v_sites[a] = sum;
+    val1 += su[a].charge * sum;
+  }
+  out->U_v_rho_N = val1;
+}
+
 void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& cf0,
                const std::vector<SitesDesc>& sites) {
   const GridArgs& g = pb.g;
@@ -1764,6 +1870,8 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
       }
     }
   }
+  for (int b = 0; b < B; ++b)
+    if (res[b].info && pb.st[b].status == RISM3D_OK) info_observables(pb, in, b, sites[b], res[b].info);
   gradients(pb, in, res, cf, sites);
 
   int worst = RISM3D_OK;

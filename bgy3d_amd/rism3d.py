@@ -79,7 +79,21 @@ class Result(ctypes.Structure):
                 ("gradient", ctypes.POINTER(ctypes.c_double)),
                 ("dmu_dsite", ctypes.POINTER(ctypes.c_double)),
                 ("adjoint_iterations", ctypes.c_int),
-                ("response", ctypes.POINTER(ctypes.c_double))]
+                ("response", ctypes.POINTER(ctypes.c_double)),
+                ("info", ctypes.c_void_p)]
+
+
+class Info(ctypes.Structure):
+    """rism3d_info: the observables print_info prints
+    (bgy3d-potential.c:341-443)."""
+    _fields_ = [("q_u", ctypes.c_double), ("q_v", ctypes.c_double),
+                ("d_n", ctypes.c_double * 3), ("d_u", ctypes.c_double * 3),
+                ("d_v", ctypes.c_double * 3),
+                ("Q_u", ctypes.c_double * 9), ("Q_v", ctypes.c_double * 9),
+                ("U_v_rho_N", ctypes.c_double), ("U_v_rho_u", ctypes.c_double),
+                ("rho_v_U_u", ctypes.c_double),
+                ("v_sites", ctypes.POINTER(ctypes.c_double)),
+                ("potential", ctypes.POINTER(ctypes.c_double))]
 
 
 EXPORTS = ("rism3d_options_default", "rism3d_solute_solve", "rism3d_batch_solve",
@@ -218,7 +232,8 @@ def _dptr(a):
 
 
 def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=False,
-                     derivatives=False, adjoint=False, response=False, chi3d=None):
+                     derivatives=False, adjoint=False, response=False, chi3d=None,
+                     info=False):
     """Solve the 3D-RISM solute problem on the GPU.
 
     Returns a dict like the reference alist: 'free-energy',
@@ -233,7 +248,10 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
     adjoint=True adds 'free-energy-dsite' (n, 6), exact
     d mu / d(x, y, z, q, sigma, epsilon) by the adjoint.
     chi3d: the --solvent-3d susceptibility of hnc3d_run_solvent ('chi-3d',
-    complex or its real part), in place of chi (no tau renormalisation)."""
+    complex or its real part), in place of chi (no tau renormalisation).
+    info=True adds 'info', the numbers print_info prints
+    (bgy3d-potential.c:341-443; see info_dict), and 'medium', the solvent
+    potential U_v on the grid (the field of the reference's Context)."""
     lib = load()
     pd = problem_data(settings)
     opt = make_options(settings)
@@ -260,6 +278,14 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
     if adjoint:
         dsite = np.full((len(su), 6), np.nan)
         res.dmu_dsite = _dptr(dsite)
+    inf = None
+    if info:
+        inf = Info()
+        vs = np.full(len(su), np.nan)
+        pot = np.empty((pd.N[2], pd.N[1], pd.N[0]))
+        inf.v_sites = _dptr(vs)
+        inf.potential = _dptr(pot)
+        res.info = ctypes.cast(ctypes.pointer(inf), ctypes.c_void_p)
     chi_p = None
     if chi3d is not None:
         c3 = np.asarray(chi3d)
@@ -304,7 +330,18 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
         out["adjoint-iterations"] = res.adjoint_iterations
     if tok is not None:
         out["RESTART"] = tok.value
+    if inf is not None and st == OK:
+        out["info"] = info_dict(inf, vs)
+        out["medium"] = pot
     return out
+
+
+def info_dict(inf, vs):
+    """rism3d_info as a dict keyed like print_info's lines."""
+    return {"q_u": inf.q_u, "q_v": inf.q_v, "d_n": list(inf.d_n), "d_u": list(inf.d_u),
+            "d_v": list(inf.d_v), "Q_u": np.array(inf.Q_u).reshape(3, 3),
+            "Q_v": np.array(inf.Q_v).reshape(3, 3), "U_v.rho_N": inf.U_v_rho_N,
+            "U_v.rho_u": inf.U_v_rho_u, "rho_v.U_u": inf.rho_v_U_u, "v_at_sites": vs}
 
 
 def hnc3d_run_solvent(solvent, settings, t_init=None, save=None):

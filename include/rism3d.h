@@ -173,7 +173,35 @@ typedef struct rism3d_result {
                           uncharged.  Either path; NaN for PY, whose
                           differential the reference lacks
                           (closures.f90:131-153)                          */
+  struct rism3d_info *info;  /* in: NULL or caller struct; out: the
+                          observables print_info prints (below)           */
 } rism3d_result;
+
+/* The post-solve observables of the solute in its solvent: info() and
+   print_info() (bgy3d-potential.c:341-491) with cage = false, as
+   hnc3d_solute_solve calls it (hnc3d.c:1774, 2325).  Fields:
+   rho_v = rho sum_i q_i g_i, the solvent charge density; U_v its potential
+   in kcal from the FFT Poisson solver (bgy3d_poisson,
+   bgy3d-poisson.c:103-190, zero mean); rho_u the Gaussian solute cores
+   (G = 1.2, bgy3d-solutes.c:555-577); U_u = eps0^-1 sum_a q_a erf(G r)/r
+   the long-range solute field; rho_N the point nuclei.  Moments are
+   bgy3d_moments (bgy3d-vec.c:573-593): about the grid centre, h^3 sum
+   v (i - N/2) h ..., Q[a][b] the second moments. */
+typedef struct rism3d_info {
+  double q_u, q_v;            /* charges of rho_u and rho_v                */
+  double d_n[3];              /* sum_a q_a x_a (dipole of the nuclei)      */
+  double d_u[3], d_v[3];      /* first moments of rho_u, rho_v             */
+  double Q_u[3][3], Q_v[3][3];/* second moments                            */
+  double U_v_rho_N;           /* <U_v|rho_N> = sum_a q_a U_v(x_a), U_v at
+                                 x_a by trigonometric interpolation
+                                 (bgy3d_fft_interp, bgy3d-fftw3.c:504-612) */
+  double U_v_rho_u;           /* <U_v|rho_u> = h^3 sum U_v rho_u           */
+  double rho_v_U_u;           /* <rho_v|U_u> = h^3 sum rho_v U_u           */
+  double *v_sites;            /* in: NULL or [n]; out: U_v(x_a)            */
+  double *potential;          /* in: NULL or [N2][N1][N0]; out: U_v on the
+                                 grid, the field of the Context the
+                                 reference returns as *medium              */
+} rism3d_info;
 
 /* Opaque restart token: the converged t (hnc3d.c:1826-1845, 2299-2316). */
 typedef struct rism3d_restart rism3d_restart;

@@ -39,6 +39,7 @@ def test_struct_layout_matches_header(tmp_path):
         "rism3d_site": [f for f, _ in R.Site._fields_],
         "rism3d_options": [f for f, _ in R.Options._fields_],
         "rism3d_result": [f for f, _ in R.Result._fields_],
+        "rism3d_info": [f for f, _ in R.Info._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
@@ -53,7 +54,7 @@ def test_struct_layout_matches_header(tmp_path):
     out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
                                                         text=True).stdout.split("\n") if l)
     py = {"rism3d_problem": R.Problem, "rism3d_site": R.Site, "rism3d_options": R.Options,
-          "rism3d_result": R.Result}
+          "rism3d_result": R.Result, "rism3d_info": R.Info}
     for st, cls in py.items():
         assert int(out[st]) == ctypes.sizeof(cls), st
         for f, _ in cls._fields_:
