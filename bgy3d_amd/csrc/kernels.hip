@@ -481,6 +481,143 @@ __device__ __forceinline__ double warp_max(double v) {
   return v;
 }
 
+// --------------------------- 3b. zstar with the spectra held in registers
+// The solute residual's k-space step (modes 0-7 of zstar, special 0) for a
+// compile-time site count MS: the z lines of one site at a time pass
+// through LDS (LINES * (N2 + 1) points), and each thread keeps its PPT
+// points of all MS sites in registers for the contraction.  LDS per
+// workgroup is 1/MS of zstar's, so a 128-byte row of kx columns (16 FP32
+// or 8 FP64) fits at high occupancy.  Same arithmetic as zstar, point by
+// point.  Grid as zstar.
+template <typename T, int N2>
+constexpr int zreg_lines() {
+  int L = 128 / (int)sizeof(cpx<T>);
+  const int pmax = sizeof(T) == 4 ? 8 : 4;  // points per thread per site
+  while (L > 2 && (N2 * L + NT - 1) / NT > pmax) L /= 2;
+  return L;
+}
+
+template <typename T, int N2, int LINES, int MS>
+__global__ void __launch_bounds__(NT)
+zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw,
+          const T* __restrict__ chi3, const T* __restrict__ tau3) {
+  constexpr int LS = lstride_for(N2);
+  constexpr int PTS = N2 * LINES;
+  constexpr int PPT = (PTS + NT - 1) / NT;
+  __shared__ cpx<T> lds[LINES * LS];
+  const int b = g.zorder ? blockIdx.z : blockIdx.x;
+  const SoluteDesc& d = sd[b];
+  if (!d.active) return;
+  const int y = g.zorder ? blockIdx.y : blockIdx.z;
+  const int kx0 = (g.zorder ? blockIdx.x : blockIdx.y) * LINES;
+  const int64_t zs = (int64_t)g.N1 * g.NH;  // z stride
+  const int64_t base = (int64_t)y * g.NH + kx0;
+  cpx<T>* W = static_cast<cpx<T>*>(d.W) + base;
+  const int nl = min(LINES, g.NX - kx0);  // valid kx columns of this tile
+  const int tid = threadIdx.x;
+  cpx<T> c[MS][PPT];
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    cpx<T>* Ws = W + s * g.K;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      const int e = tid + p * NT;
+      if (PPT * NT == PTS || e < PTS) {
+        const int z = e / LINES, l = e - z * LINES;
+        lds[l * LS + lpos(z)] = (l < nl) ? Ws[z * zs + l] : cpx<T>{T(0), T(0)};
+      }
+    }
+    __syncthreads();
+    fft_lines<T, N2, LINES, NT, false>(lds, LS, nl, tw);
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      const int e = tid + p * NT;
+      if (PPT * NT == PTS || e < PTS) {
+        const int z = e / LINES, l = e - z * LINES;
+        c[s][p] = lds[l * LS + lpos(z)];
+        if (d.mode == 3 && l < nl) Ws[z * zs + l] = c[s][p];  // forward spectrum only
+      }
+    }
+    __syncthreads();
+  }
+  if (d.mode == 3) return;
+  const bool linear = d.mode != 0;  // chi convolution only
+  const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
+  const double ky = iy * g.dky;
+  const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + base;
+#pragma unroll
+  for (int p = 0; p < PPT; ++p) {
+    const int e = tid + p * NT;
+    if (!(PPT * NT == PTS || e < PTS)) continue;
+    const int z = e / LINES, l = e - z * LINES;
+    if (l >= nl) continue;
+    const int64_t kk = base + z * zs + l;  // k-grid index of this point
+    cpx<T> fs = {T(0), T(0)};
+    if (!linear) fs = Fs[z * zs + l];
+    cpx<T> uc = {T(0), T(0)};
+    if (!g.renorm && !linear) {
+      // u_c(k) = 4 pi eps0^-1 exp(-k^2/4G^2)/k^2 F(k) s(k), 0 at k = 0
+      // (bgy3d-force.h:140-147, bgy3d-solutes.c:767-792).
+      const int iz = (z <= N2 / 2) ? z : z - N2;
+      const double k2 = kmag2((kx0 + l) * g.dkx, ky, iz * g.dkz);
+      const double shape = (k2 == 0.0) ? 0.0
+          : (4 * M_PI * EPSILON0INV) * exp(-k2 / (4 * G_COULOMB * G_COULOMB)) / k2;
+      uc = cscale(fs, T(shape));
+#pragma unroll
+      for (int s = 0; s < MS; ++s)  // h^3 C_j - beta q_j u_c  (hnc3d.c:1226-1227)
+        c[s][p] = csub(cscale(c[s][p], T(g.h3)), cscale(uc, T(g.beta * g.q[s])));
+    }
+    T chi[MS * (MS + 1) / 2];
+#pragma unroll
+    for (int q = 0; q < MS * (MS + 1) / 2; ++q) chi[q] = chi3[q * g.K + kk];
+    cpx<T> out[MS];
+#pragma unroll
+    for (int i = 0; i < MS; ++i) {
+      cpx<T> acc = {T(0), T(0)};
+#pragma unroll
+      for (int j = 0; j < MS; ++j) {
+        const T x = chi[(i >= j) ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i];
+        acc.re += x * c[j][p].re;
+        acc.im += x * c[j][p].im;
+      }
+      if (linear) {
+        out[i] = cscale(acc, T(g.a_scale));
+      } else if (g.renorm) {
+        // (h^3/L^3) sum_j chi_ij C_j - (beta/eps0/L^3) tau_i F s  (hnc3d.c:1245-1268)
+        const T tau = tau3[i * g.K + kk] * T(g.tau_scale);
+        out[i] = csub(cscale(acc, T(g.a_scale)), cscale(fs, tau));
+      } else {
+        out[i] = cscale(csub(acc, cscale(uc, T(g.beta * g.q[i]))), T(g.inv_L3));
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < MS; ++i) c[i][p] = out[i];
+  }
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    cpx<T>* Ws = W + s * g.K;
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      const int e = tid + p * NT;
+      if (PPT * NT == PTS || e < PTS) {
+        const int z = e / LINES, l = e - z * LINES;
+        lds[l * LS + lpos(z)] = c[s][p];
+      }
+    }
+    __syncthreads();
+    fft_lines<T, N2, LINES, NT, true>(lds, LS, nl, tw);
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      const int e = tid + p * NT;
+      if (PPT * NT == PTS || e < PTS) {
+        const int z = e / LINES, l = e - z * LINES;
+        if (l < nl) Ws[z * zs + l] = lds[l * LS + lpos(z)];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // -------------------------------------- 5. x c2r fused with the residual
 // t' = c2r(...), r = t' - t, u = t + lam r, and per workgroup the partial
 // max|r|, sum r^2 and dots <r, r_j> with the history (MDIIS Gram row), to

@@ -104,9 +104,34 @@ int zstar_lines(int N2, int m) {
   while (L > 2 && (size_t)m * L * lstride_for(N2) * sizeof(cpx<T>) > 64 * 1024) L /= 2;
   return L;
 }
+template <typename T, int N2>
+void launch_zreg_n(int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
+                   const cpx<T>* tw, const T* chi, const T* tau) {
+  constexpr int L = zreg_lines<T, N2>();
+  const int tiles = (NX + L - 1) / L;
+  const dim3 grid = g.zorder ? dim3(tiles, N1, B) : dim3(B, tiles, N1);
+  if (g.m == 3)
+    hipLaunchKernelGGL((zstar_reg<T, N2, L, 3>), grid, dim3(NT), 0, s, sd, g, tw, chi, tau);
+  else
+    hipLaunchKernelGGL((zstar_reg<T, N2, L, 5>), grid, dim3(NT), 0, s, sd, g, tw, chi, tau);
+}
+// zstar_reg for the solute residual with 3 or 5 solvent sites (RISM3D_ZSTAR_LDS=1 keeps
+// the all-sites-in-LDS kernel), zstar otherwise.
+bool use_zreg(const GridArgs& g) {
+  static const int lds_only = env_int("RISM3D_ZSTAR_LDS", 0);
+  return !lds_only && g.special == 0 && (g.m == 3 || g.m == 5);
+}
 template <typename T>
 void launch_zstar(int N2, int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd,
                   const GridArgs& g, const cpx<T>* tw, const T* chi, const T* tau) {
+  if (use_zreg(g)) {
+    switch (N2) {
+#define X(n) case n: launch_zreg_n<T, n>(NX, N1, B, s, sd, g, tw, chi, tau); break;
+      RISM_FOR_N(X)
+#undef X
+    }
+    return;
+  }
   const int L = zstar_lines<T>(N2, g.m);
   const size_t lds = (size_t)g.m * L * lstride_for(N2) * sizeof(cpx<T>);
   const dim3 grid = g.zorder ? dim3((NX + L - 1) / L, N1, B) : dim3(B, (NX + L - 1) / L, N1);

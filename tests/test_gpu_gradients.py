@@ -124,3 +124,28 @@ def test_response_uncharged_equals_adjoint(chi_w):
     e_tpt = np.max(np.abs(out["free-energy-gradient"] - d)) / np.max(np.abs(d))
     print(f"  TPT vs adjoint rel {e_tpt:.2e}", flush=True)
     assert e_tpt < 1e-9
+
+
+def test_autograd_solute_mu(chi_w):
+    """torch.autograd through bgy3d_amd.autograd.solute_mu: backward equals
+    the adjoint columns and central differences of the forward (h = 1e-4)
+    to 1e-4 relative."""
+    print("case autograd solute_mu", flush=True)
+    import torch
+    from bgy3d_amd import autograd as A
+    names, p0 = A.params_from_sites(_solute())
+    s = _settings("HNC")
+    p = torch.tensor(p0, requires_grad=True)
+    mu = A.solute_mu(p, names, W, s, chi_w)
+    mu.backward()
+    g = p.grad.numpy()
+    h = 1e-4
+    for a, c in ((0, 1), (2, 3), (3, 4), (1, 5)):
+        d = np.zeros_like(p0)
+        d[a, c] = h
+        with torch.no_grad():
+            fp = float(A.solute_mu(torch.tensor(p0 + d), names, W, s, chi_w))
+            fm = float(A.solute_mu(torch.tensor(p0 - d), names, W, s, chi_w))
+        fd = (fp - fm) / (2 * h)
+        print(f"  site {a} {A.COLUMNS[c]}: autograd {g[a, c]:.8f} fd {fd:.8f}", flush=True)
+        assert abs(g[a, c] - fd) <= 1e-4 * max(abs(fd), 1e-2)
