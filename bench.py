@@ -31,12 +31,12 @@ HBM_PEAK_GBS = 8000.0
 W = "water, PR-SPC/E"
 
 
-def settings_for(L_full, precision, depth=8, step=0.5, polish=3):
+def settings_for(L_full, precision, depth=8, step=0.5, polish=3, mixed_tol=1e-5):
     s = {"N": 128, "L": L_full / 2, "rho": 0.0333295, "beta": 1.6889,
          "rmax": 160.0, "nrad": 4096, "closure": "KH", "max-iter": 3000,
          "norm-tol": 1e-10, "diis-depth": depth, "diis-step": step}
     if precision == "mixed":
-        s.update({"precision": "mixed", "mixed-tol": 1e-5, "polish-iters": polish})
+        s.update({"precision": "mixed", "mixed-tol": mixed_tol, "polish-iters": polish})
     return s
 
 
@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--diis-depth", type=int, default=8)
     ap.add_argument("--diis-step", type=float, default=0.5)
     ap.add_argument("--polish", type=int, default=3)
+    ap.add_argument("--mixed-tol", type=float, default=1e-5,
+                    help="FP64-checked max|F| that ends the FP32 phase")
     ap.add_argument("--skip-fp64", action="store_true",
                     help="time the mixed batch only (profiling runs: the FP64 comparison "
                          "batch and its fields are left out)")
@@ -132,7 +134,8 @@ def main():
     solutes = S.batch(range(args.nsolutes))
     L_full = S.forced_box(solutes, N=128)
     chi = chi_solvent()
-    mixed = settings_for(L_full, "mixed", args.diis_depth, args.diis_step, args.polish)
+    mixed = settings_for(L_full, "mixed", args.diis_depth, args.diis_step, args.polish,
+                         args.mixed_tol)
     fp64 = settings_for(L_full, "fp64", args.diis_depth, args.diis_step)
 
     for _ in range(args.warmup):
@@ -166,7 +169,7 @@ def main():
         "dtype": "f32 bulk + f64 state/residual/mu",
         "data": "synthetic",
         "config": {"workload": workload, "grid": 128, "box_A": L_full, "solutes": n,
-                   "closure": "KH", "mixed_tol": 1e-5, "polish_iters": args.polish,
+                   "closure": "KH", "mixed_tol": args.mixed_tol, "polish_iters": args.polish,
                    "fp64_tol": 1e-10},
     }
     fp64_iters = None

@@ -58,11 +58,22 @@ template <typename T> __device__ __forceinline__ void st16(T* p, const Vec16<T>&
   *reinterpret_cast<Vec16<T>*>(p) = x;
 }
 
+// BF16 storage of the FP32 phase's MDIIS residual history, which enters
+// only the Gram products <r_i, r_j>: round to nearest even, four values in
+// eight bytes.  The Gram row of a new residual is formed from its rounded
+// values, so the Gram matrix stays that of the stored vectors.
+struct alignas(8) BF4 { uint32_t lo, hi; };
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float bf16_val(uint32_t h) { return __uint_as_float(h << 16); }
+
 // ------------------------------------------- 1. closure fused with x r2c
 // t = sum_j coef_j u_j (the MDIIS step; written back as the iterate) or the
 // given t, then c = closure(beta v, t), then the x-axis r2c of c.
 // Grid: x = ceil(N1 N2 / ROWS), y = site, z = solute.
-template <typename T, int N0>
+template <typename T, int N0, int H>
 __global__ void __launch_bounds__(NT)
 closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
               const cpx<T>* __restrict__ tw_half, const cpx<T>* __restrict__ tw_full) {
@@ -90,15 +101,15 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
       Vec16<T> tv;
       if (nc > 0) {
         // all history loads first, then the sums: independent loads in flight
-        Vec16<T> u[MAX_DIIS];
+        Vec16<T> u[H];
 #pragma unroll
-        for (int j = 0; j < MAX_DIIS; ++j)
+        for (int j = 0; j < H; ++j)
           if (j < nc) u[j] = ld16(static_cast<const T*>(d.u[j]) + off + e);
         double acc[VW];
 #pragma unroll
         for (int k = 0; k < VW; ++k) acc[k] = 0.0;
 #pragma unroll
-        for (int j = 0; j < MAX_DIIS; ++j)
+        for (int j = 0; j < H; ++j)
           if (j < nc) {
             const double cj = d.coef[j];
 #pragma unroll
@@ -622,7 +633,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
 // t' = c2r(...), r = t' - t, u = t + lam r, and per workgroup the partial
 // max|r|, sum r^2 and dots <r, r_j> with the history (MDIIS Gram row), to
 // part[(site * gridDim.x + blockIdx.x) * NPART + q].  Grid as closure_r2c_x.
-template <typename T, int N0>
+template <typename T, int N0, int H>
 __global__ void __launch_bounds__(NT)
 c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
             const cpx<T>* __restrict__ tw_half, const cpx<T>* __restrict__ tw_full) {
@@ -660,14 +671,15 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   fft_lines<T, M, ROWS, NT, true>(zb, LSM, vrows, tw_half);
   const int64_t off = site * g.P + (int64_t)row0 * N0;
   const T* __restrict__ t = static_cast<const T*>(d.t) + off;
-  T* __restrict__ r = static_cast<T*>(d.r) + off;
+  T* __restrict__ r = static_cast<T*>(d.r) + off;  // (BF16 slots: addressed in e below)
   T* __restrict__ un = d.unew ? static_cast<T*>(d.unew) + off : nullptr;  // null: no u
+  const bool r16 = d.r16 != 0;  // FP32 phase: r history in BF16
   const int nd = d.ndot;
   const T lam = T(d.lam);
   double mx = 0.0, ss = 0.0;
-  double dot[MAX_DIIS];
+  double dot[H];
 #pragma unroll
-  for (int j = 0; j < MAX_DIIS; ++j) dot[j] = 0.0;
+  for (int j = 0; j < H; ++j) dot[j] = 0.0;
   const int mode = d.mode;
   constexpr int VW = Vec16<T>::W;
   for (int e = threadIdx.x * VW; e < vrows * N0; e += NT * VW) {
@@ -683,10 +695,22 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
         z.v[2 * h + 1] = zz.im;
       }
     }
-    Vec16<T> rh[MAX_DIIS];
+    Vec16<T> rh[H];
 #pragma unroll
-    for (int j = 0; j < MAX_DIIS; ++j)
-      if (j < nd) rh[j] = ld16(static_cast<const T*>(d.rh[j]) + off + e);
+    for (int j = 0; j < H; ++j)
+      if (j < nd) {
+        if constexpr (sizeof(T) == 4) {
+          if (r16) {
+            const BF4 q = *reinterpret_cast<const BF4*>(static_cast<const uint16_t*>(d.rh[j]) + off + e);
+            rh[j].v[0] = bf16_val(q.lo & 0xFFFFu);
+            rh[j].v[1] = bf16_val(q.lo >> 16);
+            rh[j].v[2] = bf16_val(q.hi & 0xFFFFu);
+            rh[j].v[3] = bf16_val(q.hi >> 16);
+            continue;
+          }
+        }
+        rh[j] = ld16(static_cast<const T*>(d.rh[j]) + off + e);
+      }
     Vec16<T> rv, uv;
     if (mode == 5) {  // defect correction: F(t_b) + K dc - delta
       const Vec16<T> f = ld16(static_cast<const T*>(d.fb) + off + e);
@@ -701,17 +725,32 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
         else rv.v[k] = z.v[k];                                                       // K t
       }
     }
-    st16(r + e, rv);
 #pragma unroll
     for (int k = 0; k < VW; ++k) uv.v[k] = tv.v[k] + lam * rv.v[k];
     if (un) st16(un + e, uv);
 #pragma unroll
+    for (int k = 0; k < VW; ++k) mx = fmax(mx, fabs((double)rv.v[k]));
+    bool stored = false;
+    if constexpr (sizeof(T) == 4) {
+      if (r16) {  // store rounded, and form the Gram row from the rounded values
+        BF4 q;
+        q.lo = bf16_bits(rv.v[0]) | (bf16_bits(rv.v[1]) << 16);
+        q.hi = bf16_bits(rv.v[2]) | (bf16_bits(rv.v[3]) << 16);
+        *reinterpret_cast<BF4*>(static_cast<uint16_t*>(d.r) + off + e) = q;
+        rv.v[0] = bf16_val(q.lo & 0xFFFFu);
+        rv.v[1] = bf16_val(q.lo >> 16);
+        rv.v[2] = bf16_val(q.hi & 0xFFFFu);
+        rv.v[3] = bf16_val(q.hi >> 16);
+        stored = true;
+      }
+    }
+    if (!stored) st16(r + e, rv);
+#pragma unroll
     for (int k = 0; k < VW; ++k) {
       const double rd = (double)rv.v[k];
-      mx = fmax(mx, fabs(rd));
       ss += rd * rd;
 #pragma unroll
-      for (int j = 0; j < MAX_DIIS; ++j)
+      for (int j = 0; j < H; ++j)
         if (j < nd) dot[j] += rd * (double)rh[j].v[k];
     }
   }
@@ -720,7 +759,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   ss = warp_sum(ss);
   if (lane == 0) { red[0][w] = mx; red[1][w] = ss; }
 #pragma unroll
-  for (int j = 0; j < MAX_DIIS; ++j) {
+  for (int j = 0; j < H; ++j) {
     if (j < nd) {
       const double s = warp_sum(dot[j]);
       if (lane == 0) red[2 + j][w] = s;
@@ -806,13 +845,13 @@ add_delta(double* __restrict__ tb, const float* __restrict__ delta, int64_t n) {
 
 // Defect correction, new base t_b with F(t_b) = f64 known: x_b = -beta v +
 // t_b (PY: exp(-beta v) - 1) into xb, F(t_b) into fb; with seed (the first
-// base of a solve) also delta = 0 and the first MDIIS entry r_0 = F(t_b),
-// u_0 = 0 + lam r_0.
+// base of a solve) also delta = 0 and the first MDIIS entry r_0 = F(t_b)
+// (BF16 with r16), u_0 = 0 + lam r_0.
 __global__ void __launch_bounds__(NT)
 rebase_prep(const double* __restrict__ tb, const double* __restrict__ bv,
             const double* __restrict__ f64, int py, int seed, float lam,
             float* __restrict__ xb, float* __restrict__ fb, float* __restrict__ delta,
-            float* __restrict__ u0, float* __restrict__ r0, int64_t n) {
+            float* __restrict__ u0, void* __restrict__ r0, int r16, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * NT;
   for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) {
     const double b = bv[e];
@@ -821,7 +860,10 @@ rebase_prep(const double* __restrict__ tb, const double* __restrict__ bv,
     fb[e] = f;
     if (seed) {
       delta[e] = 0.f;
-      r0[e] = f;
+      if (r16)
+        static_cast<uint16_t*>(r0)[e] = (uint16_t)bf16_bits(f);
+      else
+        static_cast<float*>(r0)[e] = f;
       u0[e] = lam * f;
     }
   }

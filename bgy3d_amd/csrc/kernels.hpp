@@ -43,6 +43,8 @@ struct SoluteDesc {
                              //   (PY: exp(-beta v) - 1)
   const void* shift;         // mode 5: t = sum_j coef_j u_j - shift_w shift: the
   double shift_w;            //   entries of the previous base moved to this one
+  int r16;                   // FP32: r and the history rh in BF16 (Gram only)
+  int pad16_;
 };
 constexpr int NPART = 2 + MAX_DIIS;   // doubles per residual workgroup
 constexpr int INFO_NSUM = 22;         // grid sums of the info observables

@@ -49,20 +49,39 @@ bool supported_n(int n) {
 // ----------------------------------------------------- kernel dispatch
 #define RISM_FOR_N(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384)
 
+// The history loads of the closure and residual kernels are unrolled over
+// H slots held in registers: H = 8 when no solute uses more (the default
+// depth), else MAX_DIIS.
 template <typename T>
 void launch_closure_r2c(int N0, dim3 grid, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
-                        const cpx<T>* twh, const cpx<T>* twf) {
+                        const cpx<T>* twh, const cpx<T>* twf, int hist) {
+  if (hist <= 8) {
+    switch (N0) {
+#define X(n) case n: hipLaunchKernelGGL((closure_r2c_x<T, n, 8>), grid, dim3(NT), 0, s, sd, g, twh, twf); break;
+      RISM_FOR_N(X)
+#undef X
+    }
+    return;
+  }
   switch (N0) {
-#define X(n) case n: hipLaunchKernelGGL((closure_r2c_x<T, n>), grid, dim3(NT), 0, s, sd, g, twh, twf); break;
+#define X(n) case n: hipLaunchKernelGGL((closure_r2c_x<T, n, MAX_DIIS>), grid, dim3(NT), 0, s, sd, g, twh, twf); break;
     RISM_FOR_N(X)
 #undef X
   }
 }
 template <typename T>
 void launch_c2r_resid(int N0, dim3 grid, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
-                      const cpx<T>* twh, const cpx<T>* twf) {
+                      const cpx<T>* twh, const cpx<T>* twf, int hist) {
+  if (hist <= 8) {
+    switch (N0) {
+#define X(n) case n: hipLaunchKernelGGL((c2r_x_resid<T, n, 8>), grid, dim3(NT), 0, s, sd, g, twh, twf); break;
+      RISM_FOR_N(X)
+#undef X
+    }
+    return;
+  }
   switch (N0) {
-#define X(n) case n: hipLaunchKernelGGL((c2r_x_resid<T, n>), grid, dim3(NT), 0, s, sd, g, twh, twf); break;
+#define X(n) case n: hipLaunchKernelGGL((c2r_x_resid<T, n, MAX_DIIS>), grid, dim3(NT), 0, s, sd, g, twh, twf); break;
     RISM_FOR_N(X)
 #undef X
   }
@@ -290,6 +309,7 @@ struct Problem {
   double L[3], h[3];
   int D;          // DIIS depth (1 = Picard)
   int mixed;
+  int r16 = env_int("RISM3D_R_BF16", 1);  // FP32 phase: residual history in BF16
   int B;
   int64_t mP;
   hipStream_t s;
@@ -324,6 +344,14 @@ int resid_grid_x(const GridArgs& g) {
   return (g.N1 * g.N2 + rows - 1) / rows;
 }
 
+// History slots the closure and residual kernels must unroll for this batch.
+int hist_of(const std::vector<SoluteDesc>& sd) {
+  int h = 0;
+  for (const auto& d : sd)
+    if (d.active) h = std::max(h, std::max(d.ncomb, d.ndot));
+  return h;
+}
+
 void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
   const GridArgs& g = pb.g;
   HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), sd.size() * sizeof(SoluteDesc), hipMemcpyHostToDevice, pb.s));
@@ -336,20 +364,20 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
     using T = float;
     constexpr int LN = y_lines<T>();
     const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
-    mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p); mark(0, 1);
+    mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, hist_of(sd)); mark(0, 1);
     mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(1, 1);
     mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
     mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(3, 1);
-    mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p); mark(4, 1);
+    mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, hist_of(sd)); mark(4, 1);
   } else {
     using T = double;
     constexpr int LN = y_lines<T>();
     const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
-    mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p); mark(0, 1);
+    mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist_of(sd)); mark(0, 1);
     mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(1, 1);
     mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p); mark(2, 1);
     mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(3, 1);
-    mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p); mark(4, 1);
+    mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist_of(sd)); mark(4, 1);
   }
   hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, pb.d_sd.p, pb.nparts, pb.d_out.p);
   if (pf.on) {
@@ -465,7 +493,7 @@ void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
   const int D = pb.D;
   const int ch = (int)std::min<int64_t>(2048, (mP + NT - 1) / NT);
   float* U0 = reinterpret_cast<float*>(s.U.p);
-  float* R0 = reinterpret_cast<float*>(s.R.p);
+  void* R0 = s.R.p;  // slot 0, BF16 or FP32
   if (seed) {
     s.dcur = s.t32.p;
     s.dshift = s.t32b.p;
@@ -474,7 +502,7 @@ void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
   }
   hipLaunchKernelGGL(rebase_prep, dim3(ch), dim3(NT), 0, pb.s, s.t64.p, s.bv64.p, s.r64.p,
                      pb.g.closure == RISM3D_CLOSURE_PY ? 1 : 0, seed ? 1 : 0, (float)cf.lam,
-                     s.bv32.p, s.fb32.p, s.dcur, U0, R0, mP);
+                     s.bv32.p, s.fb32.p, s.dcur, U0, R0, pb.r16, mP);
   s.best32 = INFINITY;
   s.stall32 = 0;
   if (seed) {
@@ -606,7 +634,8 @@ void iterate_batch(Problem& pb, const Config& cf) {
       d.active = s.active && (s.phase32 != 0) == fp32;
       if (!d.active) continue;
       auto slotU = [&](int q) { return (void*)((char*)s.U.p + (size_t)q * mP * sz); };
-      auto slotR = [&](int q) { return (void*)((char*)s.R.p + (size_t)q * mP * sz); };
+      const size_t szr = fp32 && pb.r16 ? 2 : sz;  // BF16 residual history
+      auto slotR = [&](int q) { return (void*)((char*)s.R.p + (size_t)q * mP * szr); };
       d.t = fp32 ? (void*)s.dcur : (void*)s.t64.p;
       d.ncomb = (int)s.coef.size();
       for (int j = 0; j < d.ncomb; ++j) {
@@ -638,6 +667,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
       d.dt = s.Dt.p;
       d.gv = s.gmu.p;
       d.fb = s.fb32.p;
+      d.r16 = fp32 && pb.r16;
     }
     evaluate(pb, fp32, sd);
     HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, pb.s));
@@ -1150,7 +1180,7 @@ void info_observables(Problem& pb, const Inputs& in, int b, const SitesDesc& sit
     constexpr int LN = y_lines<T>();
     const dim3 gx(resid_grid_x(g), 1, pb.B);
     const dim3 gy((g.NX + LN - 1) / LN, g.N2, pb.B);
-    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
+    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist_of(sd));
     launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
     launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p,
                     pb.tau3.p);
@@ -1356,7 +1386,7 @@ void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& 
       constexpr int LN = y_lines<T>();
       const dim3 gx(resid_grid_x(g), g.m, B);
       const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, B);
-      launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
+      launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist_of(sd));
       launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
       launch_zstar<T>(g.N2, g.NX, g.N1, B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p);
     }
@@ -1554,7 +1584,7 @@ int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
     constexpr int LN = y_lines<T>();
     const dim3 gx(resid_grid_x(g), g.m, 1);
     const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, 1);
-    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p);
+    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist_of(sd));
     launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
     sd[0].mode = 3;
     HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), sizeof(SoluteDesc), hipMemcpyHostToDevice, pb.s));
@@ -1591,6 +1621,9 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   pb.B = in.B;
   pb.D = in.opt.solver == RISM3D_SOLVER_MDIIS ? in.opt.diis_depth : 1;
   pb.mixed = in.opt.precision != RISM3D_FP64;
+  // BF16 residual history with defect correction only: the FP32-only study
+  // iterates t itself to its floor, where the Gram rounding would show
+  if (in.opt.precision != RISM3D_MIXED) pb.r16 = 0;
   for (int d = 0; d < 3; ++d) { pb.N[d] = pd.N[d]; pb.L[d] = pd.L[d]; pb.h[d] = pd.L[d] / pd.N[d]; }
   GridArgs& g = pb.g;
   std::memset(&g, 0, sizeof g);
