@@ -57,8 +57,14 @@ template <typename T> RISM_HD T closure_f(int method, int order, T x) {
 // e^{x_b + d} - e^{x_b} as e^{max} times an expm1 in (-1, 0]: with the
 // larger exponent outside, the product neither overflows nor forms 0 * inf
 // (deep in a repulsive core x_b ~ -1e9 while d may be large).
+// Written without branches around the exponentials: with d of either sign
+// in one wavefront both sides of a branch would run.  d > 0: -e^{x_b + d}
+// expm1(-d); else e^{x_b} expm1(d) -- the same operations either way.
 template <typename T> RISM_HD T exp_diff(T xb, T d) {
-  return (d > T(0)) ? -d_exp(xb + d) * d_expm1(-d) : d_exp(xb) * d_expm1(d);
+  const T y = xb + d;
+  const T e = d_exp(d > T(0) ? y : xb);
+  const T m = d_expm1(d > T(0) ? -d : d);
+  return (d > T(0)) ? -e * m : e * m;
 }
 template <typename T>
 RISM_HD T closure_dc(int method, int order, T xb, T d) {
@@ -67,9 +73,14 @@ RISM_HD T closure_dc(int method, int order, T xb, T d) {
   if (method == 0) {
     df = exp_diff(xb, d);
   } else if (method == 1) {
-    if (xb <= T(0) && y <= T(0)) df = exp_diff(xb, d);
+    // both branches of f on the exponential side: exp_diff; both linear: d;
+    // across the branch point the linear side is exact and the other one
+    // expm1 of the smaller argument -- one exp and two expm1 per point
+    const T ed = exp_diff(xb, d);
+    const T em = d_expm1(xb < y ? xb : y);
+    if (xb <= T(0) && y <= T(0)) df = ed;
     else if (xb > T(0) && y > T(0)) df = d;
-    else df = closure_f<T>(1, 1, y) - closure_f<T>(1, 1, xb);
+    else df = (xb <= T(0)) ? y - em : em - xb;
   } else {
     if (xb < T(0) && y < T(0)) {
       df = exp_diff(xb, d);

@@ -745,13 +745,20 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
       }
     }
     if (!stored) st16(r + e, rv);
+    // products of one vector in the working type, sums across vectors in FP64
+    {
+      T s2 = T(0);
 #pragma unroll
-    for (int k = 0; k < VW; ++k) {
-      const double rd = (double)rv.v[k];
-      ss += rd * rd;
+      for (int k = 0; k < VW; ++k) s2 += rv.v[k] * rv.v[k];
+      ss += (double)s2;
 #pragma unroll
       for (int j = 0; j < H; ++j)
-        if (j < nd) dot[j] += rd * (double)rh[j].v[k];
+        if (j < nd) {
+          T pj = T(0);
+#pragma unroll
+          for (int k = 0; k < VW; ++k) pj += rv.v[k] * rh[j].v[k];
+          dot[j] += (double)pj;
+        }
     }
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;

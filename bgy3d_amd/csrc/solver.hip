@@ -389,10 +389,15 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
     for (auto& d : sd) {
       if (!d.active) continue;
       ++na;
-      // u_j in, t out (or t in), beta v in, C out
-      b0 += d.ncomb * mP * sc + mP * sc + mP * sc + mK * 2 * sc;
-      // W in, t in, r and u out, r_j in (+ F(t_b) in, defect correction)
-      b4 += mK * 2 * sc + mP * sc + 2 * mP * sc + d.ndot * mP * sc + (d.mode == 5 ? mP * sc : 0);
+      // u_j in (+ the shift of the previous base's entries), t out (or t
+      // in), beta v in, C out
+      b0 += d.ncomb * mP * sc + (d.shift_w != 0.0 ? mP * sc : 0) + mP * sc + mP * sc +
+            mK * 2 * sc;
+      // W in, t in, r and u out, r_j in (+ F(t_b) in, defect correction);
+      // r and r_j in BF16 with r16
+      const double sr = d.r16 ? 2.0 : sc;
+      b4 += mK * 2 * sc + mP * sc + mP * sr + mP * sc + d.ndot * mP * sr +
+            (d.mode == 5 ? mP * sc : 0);
     }
     const double tables = (double)(g.m * (g.m + 1) / 2 + g.m) * Kv * sc;
     const double per[5] = {b0, na * 2 * mK * 2 * sc,
