@@ -147,4 +147,65 @@ __device__ __forceinline__ void fft_lines(cpx<T>* buf, int lstride, int nl,
   }
 }
 
+// The same FFT on LINES lines stored interleaved: point p of line l at
+// buf[p * LINES + l] (LINES a power of two), the layout in which rows of
+// LINES consecutive kx columns arrive from global memory, so the copy in
+// and out is a straight copy.  Butterflies are assigned line-fastest:
+// consecutive lanes touch consecutive LDS words on every read and on every
+// write after the first stage (whose writes are R-way strided).
+template <typename T, int N, int LINES, int NT, int STAGE, bool INV>
+__device__ __forceinline__ void stockham_stage_il(cpx<T>* buf, int nl,
+                                                  const cpx<T>* __restrict__ tw) {
+  static_assert((LINES & (LINES - 1)) == 0, "LINES must be a power of two");
+  constexpr int R = radix_at(N, STAGE);
+  constexpr int NS = stage_ns(N, STAGE);
+  constexpr int NB = N / R;
+  constexpr int BF = LINES * NB;
+  constexpr int PER = (BF + NT - 1) / NT;
+  constexpr int TWS = N / (NS * R);
+  cpx<T> v[PER][4];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = tid + p * NT;
+    const int line = b & (LINES - 1);
+    if ((PER * NT == BF || b < BF) && line < nl) {
+      const int j = b / LINES;
+      const int k = j % NS;
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[p][q] = buf[(j + q * NB) * LINES + line];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          const cpx<T> w = tw[q * k * TWS];
+          v[p][q] = INV ? cmulc(v[p][q], w) : cmul(v[p][q], w);
+        }
+      }
+      dft_small<T, R, INV>(v[p]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = tid + p * NT;
+    const int line = b & (LINES - 1);
+    if ((PER * NT == BF || b < BF) && line < nl) {
+      const int j = b / LINES;
+      const int k = j % NS;
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[(base + q * NS) * LINES + line] = v[p][q];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int LINES, int NT, bool INV, int S = 0>
+__device__ __forceinline__ void fft_lines_il(cpx<T>* buf, int nl, const cpx<T>* __restrict__ tw) {
+  if constexpr (S < num_stages(N)) {
+    stockham_stage_il<T, N, LINES, NT, S, INV>(buf, nl, tw);
+    fft_lines_il<T, N, LINES, NT, INV, S + 1>(buf, nl, tw);
+  }
+}
+
 }  // namespace rism

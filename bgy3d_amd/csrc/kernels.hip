@@ -51,11 +51,59 @@ template <typename T> struct alignas(16) Vec16 {
   static constexpr int W = 16 / sizeof(T);
   T v[W];
 };
+// Global-memory accessors.  The array pointers come out of descriptors in
+// memory, so the compiler sees generic pointers and emits flat accesses,
+// which also count against the LDS wait counter: every wait for an LDS
+// access would then wait for the outstanding HBM loads too.  These casts to
+// address space 1 (through native vector types; aggregate copies lose the
+// address space) give global_load / global_store.
+template <typename T> struct VecOf;
+template <> struct VecOf<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef float v2 __attribute__((ext_vector_type(2)));
+};
+template <> struct VecOf<double> {
+  typedef double v4 __attribute__((ext_vector_type(2)));  // 16 bytes
+  typedef double v2 __attribute__((ext_vector_type(2)));
+};
 template <typename T> __device__ __forceinline__ Vec16<T> ld16(const T* p) {
-  return *reinterpret_cast<const Vec16<T>*>(p);
+  typedef typename VecOf<T>::v4 V;
+  typedef __attribute__((address_space(1))) const V GV;
+  const V x = *(GV*)p;
+  Vec16<T> r;
+#pragma unroll
+  for (int k = 0; k < Vec16<T>::W; ++k) r.v[k] = x[k];
+  return r;
 }
-template <typename T> __device__ __forceinline__ void st16(T* p, const Vec16<T>& x) {
-  *reinterpret_cast<Vec16<T>*>(p) = x;
+template <typename T> __device__ __forceinline__ void st16(T* p, const Vec16<T>& r) {
+  typedef typename VecOf<T>::v4 V;
+  typedef __attribute__((address_space(1))) V GV;
+  V x;
+#pragma unroll
+  for (int k = 0; k < Vec16<T>::W; ++k) x[k] = r.v[k];
+  *(GV*)p = x;
+}
+template <typename T> __device__ __forceinline__ cpx<T> ldc(const cpx<T>* p) {
+  typedef typename VecOf<T>::v2 V;
+  typedef __attribute__((address_space(1))) const V GV;
+  const V x = *(GV*)p;
+  return cpx<T>{x[0], x[1]};
+}
+template <typename T> __device__ __forceinline__ void stc(cpx<T>* p, cpx<T> c) {
+  typedef typename VecOf<T>::v2 V;
+  typedef __attribute__((address_space(1))) V GV;
+  V x;
+  x[0] = c.re;
+  x[1] = c.im;
+  *(GV*)p = x;
+}
+template <typename X> __device__ __forceinline__ X gld(const X* p) {  // scalars
+  typedef __attribute__((address_space(1))) const X GX;
+  return *(GX*)p;
+}
+template <typename X> __device__ __forceinline__ void gst(X* p, X v) {
+  typedef __attribute__((address_space(1))) X GX;
+  *(GX*)p = v;
 }
 
 // BF16 storage of the FP32 phase's MDIIS residual history, which enters
@@ -68,6 +116,19 @@ __device__ __forceinline__ uint32_t bf16_bits(float f) {
   return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
 }
 __device__ __forceinline__ float bf16_val(uint32_t h) { return __uint_as_float(h << 16); }
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ BF4 ldbf4(const uint16_t* p) {
+  typedef __attribute__((address_space(1))) const u32x2 G;
+  const u32x2 x = *(G*)p;
+  return BF4{x[0], x[1]};
+}
+__device__ __forceinline__ void stbf4(uint16_t* p, BF4 q) {
+  typedef __attribute__((address_space(1))) u32x2 G;
+  u32x2 x;
+  x[0] = q.lo;
+  x[1] = q.hi;
+  *(G*)p = x;
+}
 
 // ------------------------------------------- 1. closure fused with x r2c
 // t = sum_j coef_j u_j (the MDIIS step; written back as the iterate) or the
@@ -134,9 +195,9 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
           c.v[k] = (g.closure == 2) ? b.v[k] * tv.v[k]
                                     : closure_dc<T>(g.closure, g.pse_order, b.v[k], tv.v[k]);
         else if (d.mode == 6)  // J v: dc = (dc/dt) v (compute_c1, hnc3d.c:1307)
-          c.v[k] = T(d.dt[off + e + k] * (double)tv.v[k]);
+          c.v[k] = T(gld(d.dt + off + e + k) * (double)tv.v[k]);
         else if (d.mode == 7)  // flag form: dc + v (hnc3d.c:1310-1320)
-          c.v[k] = T((d.dt[off + e + k] + 1.0) * (double)tv.v[k]);
+          c.v[k] = T((gld(d.dt + off + e + k) + 1.0) * (double)tv.v[k]);
         else if (d.mode == 10)  // rho h for the susceptibility (hnc3d.c:1006-1030)
           c.v[k] = T(g.rho) * (closure_c<T>(g.closure, g.pse_order, b.v[k], tv.v[k]) + tv.v[k]);
         else c.v[k] = tv.v[k];
@@ -159,7 +220,7 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
     const cpx<T> zm = cconj(lds[row * LSM + lpos(k == 0 ? 0 : M - k)]);
     const cpx<T> s = cadd(zk, zm);
     const cpx<T> dd = rot90<T, false>(cmul(tw_full[k], csub(zk, zm)));  // -i w (Zk - conj Zm)
-    W[row * g.NH + k] = cpx<T>{T(0.5) * (s.re + dd.re), T(0.5) * (s.im + dd.im)};
+    stc(W + row * g.NH + k, cpx<T>{T(0.5) * (s.re + dd.re), T(0.5) * (s.im + dd.im)});
   }
 }
 
@@ -168,23 +229,40 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
 template <typename T, int N1, int LINES, bool INV>
 __global__ void __launch_bounds__(NT)
 fft_y(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw) {
-  constexpr int LS = lstride_for(N1);
-  __shared__ cpx<T> lds[LINES * LS];
+  __shared__ cpx<T> lds[LINES * N1];  // interleaved: point y of column l at y * LINES + l
   const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
   const int site = blockIdx.y / g.N2;
   const int z = blockIdx.y - site * g.N2;
   const int kx0 = blockIdx.x * LINES;
   cpx<T>* W = static_cast<cpx<T>*>(d.W) + site * g.K + (int64_t)z * g.N1 * g.NH + kx0;
-  for (int e = threadIdx.x; e < N1 * LINES; e += NT) {
-    const int y = e / LINES, l = e - y * LINES;
-    lds[l * LS + lpos(y)] = (kx0 + l < g.NX) ? W[y * g.NH + l] : cpx<T>{T(0), T(0)};
+  {
+    // all loads first (the row padding keeps every column of a tile in
+    // bounds; columns past NX are zeroed after the load)
+    constexpr int PTS = N1 * LINES, PT = (PTS + NT - 1) / NT;
+    cpx<T> v[PT];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      const int e = threadIdx.x + p * NT;
+      if (PT * NT == PTS || e < PTS) {
+        const int y = e / LINES, l = e - y * LINES;
+        v[p] = ldc(W + y * g.NH + l);
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      const int e = threadIdx.x + p * NT;
+      if (PT * NT == PTS || e < PTS) {
+        const int l = e & (LINES - 1);
+        lds[e] = (kx0 + l < g.NX) ? v[p] : cpx<T>{T(0), T(0)};
+      }
+    }
   }
   __syncthreads();
-  fft_lines<T, N1, LINES, NT, INV>(lds, LS, min(LINES, g.NX - kx0), tw);
+  fft_lines_il<T, N1, LINES, NT, INV>(lds, min(LINES, g.NX - kx0), tw);
   for (int e = threadIdx.x; e < N1 * LINES; e += NT) {
     const int y = e / LINES, l = e - y * LINES;
-    if (kx0 + l < g.NX) W[y * g.NH + l] = lds[l * LS + lpos(y)];
+    if (kx0 + l < g.NX) stc(W + y * g.NH + l, lds[e]);
   }
 }
 
@@ -495,7 +573,7 @@ __device__ __forceinline__ double warp_max(double v) {
 // --------------------------- 3b. zstar with the spectra held in registers
 // The solute residual's k-space step (modes 0-7 of zstar, special 0) for a
 // compile-time site count MS: the z lines of one site at a time pass
-// through LDS (LINES * (N2 + 1) points), and each thread keeps its PPT
+// through LDS (LINES * N2 points, interleaved), and each thread keeps its PPT
 // points of all MS sites in registers for the contraction.  LDS per
 // workgroup is 1/MS of zstar's, so a 128-byte row of kx columns (16 FP32
 // or 8 FP64) fits at high occupancy.  Same arithmetic as zstar, point by
@@ -512,10 +590,9 @@ template <typename T, int N2, int LINES, int MS>
 __global__ void __launch_bounds__(NT)
 zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw,
           const T* __restrict__ chi3, const T* __restrict__ tau3) {
-  constexpr int LS = lstride_for(N2);
   constexpr int PTS = N2 * LINES;
   constexpr int PPT = (PTS + NT - 1) / NT;
-  __shared__ cpx<T> lds[LINES * LS];
+  __shared__ cpx<T> lds[PTS];  // interleaved: point z of column l at z * LINES + l
   const int b = g.zorder ? blockIdx.z : blockIdx.x;
   const SoluteDesc& d = sd[b];
   if (!d.active) return;
@@ -530,23 +607,33 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
     cpx<T>* Ws = W + s * g.K;
+    {
+      // all loads first; the row padding keeps every column of a tile in
+      // bounds, columns past NX are zeroed after the load
+      cpx<T> v[PPT];
 #pragma unroll
-    for (int p = 0; p < PPT; ++p) {
-      const int e = tid + p * NT;
-      if (PPT * NT == PTS || e < PTS) {
-        const int z = e / LINES, l = e - z * LINES;
-        lds[l * LS + lpos(z)] = (l < nl) ? Ws[z * zs + l] : cpx<T>{T(0), T(0)};
+      for (int p = 0; p < PPT; ++p) {
+        const int e = tid + p * NT;
+        if (PPT * NT == PTS || e < PTS) {
+          const int z = e / LINES, l = e - z * LINES;
+          v[p] = ldc(Ws + z * zs + l);
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        const int e = tid + p * NT;
+        if (PPT * NT == PTS || e < PTS) lds[e] = ((e & (LINES - 1)) < nl) ? v[p] : cpx<T>{T(0), T(0)};
       }
     }
     __syncthreads();
-    fft_lines<T, N2, LINES, NT, false>(lds, LS, nl, tw);
+    fft_lines_il<T, N2, LINES, NT, false>(lds, nl, tw);
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
       const int e = tid + p * NT;
       if (PPT * NT == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
-        c[s][p] = lds[l * LS + lpos(z)];
-        if (d.mode == 3 && l < nl) Ws[z * zs + l] = c[s][p];  // forward spectrum only
+        c[s][p] = lds[e];
+        if (d.mode == 3 && l < nl) stc(Ws + z * zs + l, c[s][p]);  // forward spectrum only
       }
     }
     __syncthreads();
@@ -564,7 +651,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
     if (l >= nl) continue;
     const int64_t kk = base + z * zs + l;  // k-grid index of this point
     cpx<T> fs = {T(0), T(0)};
-    if (!linear) fs = Fs[z * zs + l];
+    if (!linear) fs = ldc(Fs + z * zs + l);
     cpx<T> uc = {T(0), T(0)};
     if (!g.renorm && !linear) {
       // u_c(k) = 4 pi eps0^-1 exp(-k^2/4G^2)/k^2 F(k) s(k), 0 at k = 0
@@ -580,7 +667,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
     }
     T chi[MS * (MS + 1) / 2];
 #pragma unroll
-    for (int q = 0; q < MS * (MS + 1) / 2; ++q) chi[q] = chi3[q * g.K + kk];
+    for (int q = 0; q < MS * (MS + 1) / 2; ++q) chi[q] = gld(chi3 + q * g.K + kk);
     cpx<T> out[MS];
 #pragma unroll
     for (int i = 0; i < MS; ++i) {
@@ -595,7 +682,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
         out[i] = cscale(acc, T(g.a_scale));
       } else if (g.renorm) {
         // (h^3/L^3) sum_j chi_ij C_j - (beta/eps0/L^3) tau_i F s  (hnc3d.c:1245-1268)
-        const T tau = tau3[i * g.K + kk] * T(g.tau_scale);
+        const T tau = gld(tau3 + i * g.K + kk) * T(g.tau_scale);
         out[i] = csub(cscale(acc, T(g.a_scale)), cscale(fs, tau));
       } else {
         out[i] = cscale(csub(acc, cscale(uc, T(g.beta * g.q[i]))), T(g.inv_L3));
@@ -612,17 +699,17 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
       const int e = tid + p * NT;
       if (PPT * NT == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
-        lds[l * LS + lpos(z)] = c[s][p];
+        lds[e] = c[s][p];
       }
     }
     __syncthreads();
-    fft_lines<T, N2, LINES, NT, true>(lds, LS, nl, tw);
+    fft_lines_il<T, N2, LINES, NT, true>(lds, nl, tw);
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
       const int e = tid + p * NT;
       if (PPT * NT == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
-        if (l < nl) Ws[z * zs + l] = lds[l * LS + lpos(z)];
+        if (l < nl) stc(Ws + z * zs + l, lds[e]);
       }
     }
     __syncthreads();
@@ -650,10 +737,25 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   const int row0 = blockIdx.x * ROWS;
   const int vrows = min(ROWS, nrows - row0);
   const cpx<T>* W = static_cast<const cpx<T>*>(d.W) + site * g.K + (int64_t)row0 * g.NH;
-  for (int e = threadIdx.x; e < ROWS * (M + 1); e += NT) {
-    const int row = e / (M + 1);
-    const int k = e - row * (M + 1);
-    xb[e] = (row < vrows) ? W[row * g.NH + k] : cpx<T>{T(0), T(0)};
+  {
+    // all loads first; rows past the last one read the last row's address
+    // and are zeroed
+    constexpr int PTS = ROWS * (M + 1), PT = (PTS + NT - 1) / NT;
+    cpx<T> v[PT];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      const int e = threadIdx.x + p * NT;
+      if (PT * NT == PTS || e < PTS) {
+        const int row = e / (M + 1);
+        const int k = e - row * (M + 1);
+        v[p] = ldc(W + min(row, vrows - 1) * g.NH + k);
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      const int e = threadIdx.x + p * NT;
+      if (PT * NT == PTS || e < PTS) xb[e] = (e / (M + 1) < vrows) ? v[p] : cpx<T>{T(0), T(0)};
+    }
   }
   __syncthreads();
   for (int e = threadIdx.x; e < ROWS * M; e += NT) {
@@ -701,7 +803,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
       if (j < nd) {
         if constexpr (sizeof(T) == 4) {
           if (r16) {
-            const BF4 q = *reinterpret_cast<const BF4*>(static_cast<const uint16_t*>(d.rh[j]) + off + e);
+            const BF4 q = ldbf4(static_cast<const uint16_t*>(d.rh[j]) + off + e);
             rh[j].v[0] = bf16_val(q.lo & 0xFFFFu);
             rh[j].v[1] = bf16_val(q.lo >> 16);
             rh[j].v[2] = bf16_val(q.hi & 0xFFFFu);
@@ -721,7 +823,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
       for (int k = 0; k < VW; ++k) {
         if (mode == 0 || mode == 6) rv.v[k] = z.v[k] - tv.v[k];                     // F(t), J v
         else if (mode == 1)                                                          // adjoint
-          rv.v[k] = T(d.dt[off + e + k]) * z.v[k] + T(d.gv[off + e + k]) - tv.v[k];
+          rv.v[k] = T(gld(d.dt + off + e + k)) * z.v[k] + T(gld(d.gv + off + e + k)) - tv.v[k];
         else rv.v[k] = z.v[k];                                                       // K t
       }
     }
@@ -736,7 +838,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
         BF4 q;
         q.lo = bf16_bits(rv.v[0]) | (bf16_bits(rv.v[1]) << 16);
         q.hi = bf16_bits(rv.v[2]) | (bf16_bits(rv.v[3]) << 16);
-        *reinterpret_cast<BF4*>(static_cast<uint16_t*>(d.r) + off + e) = q;
+        stbf4(static_cast<uint16_t*>(d.r) + off + e, q);
         rv.v[0] = bf16_val(q.lo & 0xFFFFu);
         rv.v[1] = bf16_val(q.lo >> 16);
         rv.v[2] = bf16_val(q.hi & 0xFFFFu);
@@ -777,7 +879,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
     const int q = threadIdx.x;
     double a = red[q][0];
     for (int i = 1; i < NT / 64; ++i) a = (q == 0) ? fmax(a, red[q][i]) : a + red[q][i];
-    d.part[((int64_t)site * gridDim.x + blockIdx.x) * NPART + q] = a;
+    gst(d.part + ((int64_t)site * gridDim.x + blockIdx.x) * NPART + q, a);
   }
 }
 
