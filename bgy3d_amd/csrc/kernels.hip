@@ -160,12 +160,15 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
     for (int k = 0; k < VW; ++k) c.v[k] = T(0);
     if (e < nvalid) {  // nvalid is a multiple of N0: whole vectors
       Vec16<T> tv;
+      const Vec16<T> b = ld16(bv + e);
       if (nc > 0) {
-        // all history loads first, then the sums: independent loads in flight
+        // all loads first, then the sums: independent loads in flight
         Vec16<T> u[H];
 #pragma unroll
         for (int j = 0; j < H; ++j)
           if (j < nc) u[j] = ld16(static_cast<const T*>(d.u[j]) + off + e);
+        Vec16<T> sh;
+        if (d.shift_w != 0.0) sh = ld16(static_cast<const T*>(d.shift) + off + e);
         double acc[VW];
 #pragma unroll
         for (int k = 0; k < VW; ++k) acc[k] = 0.0;
@@ -177,7 +180,6 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
             for (int k = 0; k < VW; ++k) acc[k] += cj * (double)u[j].v[k];
           }
         if (d.shift_w != 0.0) {
-          const Vec16<T> sh = ld16(static_cast<const T*>(d.shift) + off + e);
 #pragma unroll
           for (int k = 0; k < VW; ++k) acc[k] -= d.shift_w * (double)sh.v[k];
         }
@@ -187,7 +189,6 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
       } else {
         tv = ld16(t + e);
       }
-      const Vec16<T> b = ld16(bv + e);
 #pragma unroll
       for (int k = 0; k < VW; ++k) {
         if (d.mode == 0) c.v[k] = closure_c<T>(g.closure, g.pse_order, b.v[k], tv.v[k]);
@@ -797,6 +798,8 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
         z.v[2 * h + 1] = zz.im;
       }
     }
+    Vec16<T> f;  // F(t_b), defect correction
+    if (mode == 5) f = ld16(static_cast<const T*>(d.fb) + off + e);
     Vec16<T> rh[H];
 #pragma unroll
     for (int j = 0; j < H; ++j)
@@ -815,7 +818,6 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
       }
     Vec16<T> rv, uv;
     if (mode == 5) {  // defect correction: F(t_b) + K dc - delta
-      const Vec16<T> f = ld16(static_cast<const T*>(d.fb) + off + e);
 #pragma unroll
       for (int k = 0; k < VW; ++k) rv.v[k] = f.v[k] + z.v[k] - tv.v[k];
     } else {
