@@ -11,7 +11,9 @@ A step is one batched solve of all 64 solutes to convergence (setup of the
 solute fields, MDIIS iterations, mu, all on the GPU).  Mixed precision:
 FP32 FFT / closure / contraction until max|F| <= 1e-5, then 3 FP64 polish
 iterations.  One FP64 batch (max|F| <= 1e-10) is timed alongside for the
-speed ratio and |dmu|.
+speed ratio and |dmu|.  `natural_grids` reports the same 64 solutes on
+their own grid sizes (14 on 64^3, 49 on 96^3, 1 on 128^3), one batch per
+size.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -31,8 +33,8 @@ HBM_PEAK_GBS = 8000.0
 W = "water, PR-SPC/E"
 
 
-def settings_for(L_full, precision, depth=8, step=0.5, polish=3, mixed_tol=1e-5):
-    s = {"N": 128, "L": L_full / 2, "rho": 0.0333295, "beta": 1.6889,
+def settings_for(L_full, precision, depth=8, step=0.5, polish=3, mixed_tol=1e-5, N=128):
+    s = {"N": N, "L": L_full / 2, "rho": 0.0333295, "beta": 1.6889,
          "rmax": 160.0, "nrad": 4096, "closure": "KH", "max-iter": 3000,
          "norm-tol": 1e-10, "diis-depth": depth, "diis-step": step}
     if precision == "mixed":
@@ -92,6 +94,48 @@ def cpu_baseline(solute, L_full, chi, fp64_iters, n_eval=4):
                        f"(mean FP64 MDIIS count on the GPU)")}
 
 
+def natural_grids(solutes, chi, args):
+    """The same solutes on their own grids (synthetic.box_for: the smallest
+    of 64^3 / 96^3 / 128^3 at <= 0.35 A spacing with 8 A padding), one
+    batched launch sequence per grid size in a box that pads every member:
+    solves/s over the whole set, mixed and FP64, and |dmu| between them.
+    One untimed and one timed pass."""
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd import synthetic as S
+    groups = {}
+    for i, s in enumerate(solutes):
+        groups.setdefault(S.box_for(s)[0], []).append(i)
+    plan = []
+    for N in sorted(groups):
+        members = [solutes[i] for i in groups[N]]
+        box = S.forced_box(members, N=N)
+        plan.append((N, groups[N], members, box))
+
+    def run(precision):
+        out = [None] * len(solutes)
+        for N, idx, members, box in plan:
+            st = settings_for(box, precision, args.diis_depth, args.diis_step, args.polish,
+                              args.mixed_tol, N=N)
+            for i, r in zip(idx, R.batch_solve(members, W, st, chi)):
+                out[i] = r
+        return out
+
+    run("mixed")
+    t0 = time.perf_counter()
+    mixed = run("mixed")
+    t_mixed = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fp64 = run("fp64")
+    t_fp64 = time.perf_counter() - t0
+    dmu = [abs(a["free-energy"] - b["free-energy"]) for a, b in zip(mixed, fp64)]
+    n = len(solutes)
+    return {"groups": {f"{N}^3 (box {box} A)": len(idx) for N, idx, _, box in plan},
+            "mixed_solves_per_s": n / t_mixed, "fp64_solves_per_s": n / t_fp64,
+            "mixed_over_fp64": t_fp64 / t_mixed, "max_abs_dmu_kcal": max(dmu),
+            "all_converged": all(a["status"] == "OK" and b["status"] == "OK"
+                                 for a, b in zip(mixed, fp64))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +146,8 @@ def main():
     ap.add_argument("--diis-depth", type=int, default=8)
     ap.add_argument("--diis-step", type=float, default=0.5)
     ap.add_argument("--polish", type=int, default=3)
+    ap.add_argument("--skip-natural", action="store_true",
+                    help="skip the natural-grid report (solutes on 64^3 / 96^3 / 128^3)")
     ap.add_argument("--mixed-tol", type=float, default=1e-5,
                     help="FP64-checked max|F| that ends the FP32 phase")
     ap.add_argument("--skip-fp64", action="store_true",
@@ -207,6 +253,8 @@ def main():
                         "bytes_per_launch": v[1] / max(v[2], 1),
                         "launches": v[2]} for k, v in prof.items()},
     })
+    if not args.skip_natural:
+        line["natural_grids"] = natural_grids(solutes, chi, args)
     if not args.no_cpu_baseline:
         iters = fp64_iters if fp64_iters is not None else line["mean_iters_mixed"]
         line["cpu_baseline"] = cpu_baseline(solutes[0], L_full, chi, iters)
