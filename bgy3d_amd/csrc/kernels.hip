@@ -529,12 +529,13 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     }
     for (int i = 0; i < m; ++i) {
       cpx<T> acc = {T(0), T(0)};
-      for (int j = 0; j < m; ++j) {
+      for (int j = 0; j < m; ++j) {  // sum_j S_ij sig_j c_j / sig_i (sig = 1: one density)
         const int pi_ = (i >= j) ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
-        const T chi = chi3[pi_ * g.K + kk];
+        const T chi = chi3[pi_ * g.K + kk] * T(g.sig[j]);
         acc.re += chi * c[j].re;
         acc.im += chi * c[j].im;
       }
+      acc = cscale(acc, T(1.0 / g.sig[i]));
       cpx<T> out;
       if (linear) {
         out = cscale(acc, T(g.a_scale));
@@ -674,11 +675,12 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
     for (int i = 0; i < MS; ++i) {
       cpx<T> acc = {T(0), T(0)};
 #pragma unroll
-      for (int j = 0; j < MS; ++j) {
-        const T x = chi[(i >= j) ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i];
+      for (int j = 0; j < MS; ++j) {  // sum_j S_ij sig_j c_j / sig_i (sig = 1: one density)
+        const T x = chi[(i >= j) ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i] * T(g.sig[j]);
         acc.re += x * c[j][p].re;
         acc.im += x * c[j][p].im;
       }
+      acc = cscale(acc, T(1.0 / g.sig[i]));
       if (linear) {
         out[i] = cscale(acc, T(g.a_scale));
       } else if (g.renorm) {
@@ -1121,11 +1123,12 @@ mu_density(const SitesDesc* __restrict__ sites, GridArgs g, const double* const*
       const double h = c + ti;
       const double xx = -bvi + ti;
       const double cl = -g.beta * g.q[i] * uc;
-      v[0] += form_mu(g.closure, g.pse_order, xx, h, c, cl);
-      v[1] += form_mu(0, 0, xx, h, c, cl);
-      v[2] += form_mu(1, 1, xx, h, c, cl);
-      v[3] += form_mu(2, 0, xx, h, c, cl);
-      v[4] += c;
+      const double w = g.rw[i];  // rho_i / rho (1: one density)
+      v[0] += w * form_mu(g.closure, g.pse_order, xx, h, c, cl);
+      v[1] += w * form_mu(0, 0, xx, h, c, cl);
+      v[2] += w * form_mu(1, 1, xx, h, c, cl);
+      v[3] += w * form_mu(2, 0, xx, h, c, cl);
+      v[4] += w * c;
       if (go) go[idx] = h + 1.0;
     }
 #pragma unroll
@@ -1608,7 +1611,7 @@ info_rho_v(GridArgs g, const double* __restrict__ t, const double* __restrict__ 
     for (int i = 0; i < g.m; ++i) {
       const int64_t idx = i * g.P + e;
       const double c = closure_c<double>(g.closure, g.pse_order, bv[idx], t[idx]);
-      acc += (g.q[i] * rho) * ((c + t[idx]) + 1.0);
+      acc += (g.q[i] * rho * g.rw[i]) * ((c + t[idx]) + 1.0);
     }
     rhov[e] = acc;
   }

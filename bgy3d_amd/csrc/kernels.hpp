@@ -76,6 +76,11 @@ struct GridArgs {
   int special;
   int msol;            // solvent sites of the solvent-solvent mode
   double rho;          // solvent number density
+  // mixtures (per-site densities rho_i): sig_i = sqrt(rho_i / rho) scales the
+  // symmetric table S = D^-1/2 chi D^1/2 back to h_i = sum_j c_j chi_ji,
+  // rw_i = rho_i / rho weights mu; both 1 for one density
+  double sig[MAX_SITES];
+  double rw[MAX_SITES];
 };
 
 // Solute sites for the setup and mu kernels, per solute.

@@ -257,11 +257,13 @@ std::vector<double> local_coords(const std::vector<Site3>& s) {
 struct Rism1D {
   int m, n, method;
   double rmax, beta, rho, eps, dr, dk;
+  std::vector<double> rs;  // number density of each site (rho for one molecule)
   std::vector<double> r, k;
   std::vector<double> vr, vk, wk, xk, W;  // [m][m][n]
   Dst4 dst;
   Rism1D(int m_, int n_, int method_, double rmax_, double beta_, double rho_, double eps_)
-      : m(m_), n(n_), method(method_), rmax(rmax_), beta(beta_), rho(rho_), eps(eps_), dst(n_) {}
+      : m(m_), n(n_), method(method_), rmax(rmax_), beta(beta_), rho(rho_), eps(eps_), rs(m_, rho_),
+        dst(n_) {}
 
   size_t at(int i, int j, int p) const { return ((size_t)i * m + j) * n + p; }
 
@@ -273,7 +275,7 @@ struct Rism1D {
     for (int p = 0; p < n; ++p) g[p] = g[p] / (2 * p + 1) * s;
   }
 
-  // t(k) = OZ(c(k)) per k; also keeps A^-1 W and W + rho H for J.v
+  // t(k) = OZ(c(k)) per k: H = (1 - W C D)^-1 W C W, D = diag(rs)
   void oz(const std::vector<double>& c, std::vector<double>& t) const {
     std::vector<double> WC(m * m), A(m * m), H(m * m);
     for (int p = 0; p < n; ++p) {
@@ -285,7 +287,7 @@ struct Rism1D {
         }
       for (int i = 0; i < m; ++i)
         for (int j = 0; j < m; ++j) {
-          A[i * m + j] = (i == j ? 1.0 : 0.0) - WC[i * m + j] * rho;
+          A[i * m + j] = (i == j ? 1.0 : 0.0) - WC[i * m + j] * rs[j];  // W C D
           double s = 0;
           for (int q = 0; q < m; ++q) s += WC[i * m + q] * W[at(q, j, p)];
           H[i * m + j] = s;
@@ -296,7 +298,7 @@ struct Rism1D {
         for (int j = 0; j < m; ++j) t[at(i, j, p)] = H[i * m + j] - c[at(i, j, p)];
     }
   }
-  // dT = A^-1 W dC (W + rho H) - dC
+  // dT = A^-1 W dC (W + D H) - dC
   void oz1(const std::vector<double>& c, const std::vector<double>& dc, std::vector<double>& dt) const {
     std::vector<double> WC(m * m), A(m * m), H(m * m), X(m * m), Y(m * m);
     for (int p = 0; p < n; ++p) {
@@ -308,7 +310,7 @@ struct Rism1D {
         }
       for (int i = 0; i < m; ++i)
         for (int j = 0; j < m; ++j) {
-          A[i * m + j] = (i == j ? 1.0 : 0.0) - WC[i * m + j] * rho;
+          A[i * m + j] = (i == j ? 1.0 : 0.0) - WC[i * m + j] * rs[j];  // W C D
           double s = 0;
           for (int q = 0; q < m; ++q) s += WC[i * m + q] * W[at(q, j, p)];
           H[i * m + j] = s;
@@ -325,7 +327,7 @@ struct Rism1D {
       for (int i = 0; i < m; ++i)
         for (int j = 0; j < m; ++j) {
           double s = 0;
-          for (int q = 0; q < m; ++q) s += X[i * m + q] * (W[at(q, j, p)] + rho * H[q * m + j]);
+          for (int q = 0; q < m; ++q) s += X[i * m + q] * (W[at(q, j, p)] + rs[q] * H[q * m + j]);
           Y[i * m + j] = s;
         }
       AA = A;
@@ -455,12 +457,22 @@ void gmres(Op A, const std::vector<double>& b, std::vector<double>& x, double rt
 
 }  // namespace
 
-extern "C" int rism3d_solvent_chi_internal(const rism3d_problem* pd, int m,
-                                           const rism3d_site* solvent, double dielectric,
-                                           double* chi_out) {
+namespace {
+// One molecule at pd->rho with DRISM (molecule == nullptr), or a mixture
+// (XRISM): molecule[i] the molecule of site i, rho_sites[i] its density.
+int solvent_chi(const rism3d_problem* pd, int m, const rism3d_site* solvent, double dielectric,
+                const int* molecule, const double* rho_sites, double* chi_out) {
   if (!pd || !solvent || !chi_out || m < 1 || m > 8 || pd->nrad < 4 || !(pd->rmax > 0) ||
       !(pd->beta > 0) || !(pd->rho > 0) || pd->closure < 0 || pd->closure > 10)
     return RISM3D_BAD_ARG;
+  if (molecule) {
+    if (!rho_sites || dielectric != 0) return RISM3D_BAD_ARG;  // mixtures: XRISM only
+    for (int i = 0; i < m; ++i) {
+      if (!(rho_sites[i] > 0)) return RISM3D_BAD_ARG;
+      for (int j = 0; j < m; ++j)  // sites of one molecule share its density
+        if (molecule[i] == molecule[j] && rho_sites[i] != rho_sites[j]) return RISM3D_BAD_ARG;
+    }
+  }
   const int n = pd->nrad;
   std::vector<Site3> s(m);
   for (int i = 0; i < m; ++i) {
@@ -470,6 +482,8 @@ extern "C" int rism3d_solvent_chi_internal(const rism3d_problem* pd, int m,
     s[i].q = solvent[i].charge;
   }
   Rism1D R(m, n, pd->closure, pd->rmax, pd->beta, pd->rho, dielectric);
+  if (molecule)
+    for (int i = 0; i < m; ++i) R.rs[i] = rho_sites[i];
   R.dr = pd->rmax / n;
   R.dk = PI / pd->rmax;
   R.r.resize(n);
@@ -500,7 +514,7 @@ extern "C" int rism3d_solvent_chi_internal(const rism3d_problem* pd, int m,
         }
         R.vr[R.at(i, j, p)] = v;
         R.vk[R.at(i, j, p)] = EPS0INV * qab * 4 * PI * std::exp(-kk * kk / (4 * ALPHA * ALPHA)) / (kk * kk);
-        R.wk[R.at(i, j, p)] = sj0(kk * rab);
+        R.wk[R.at(i, j, p)] = (molecule && molecule[i] != molecule[j]) ? 0.0 : sj0(kk * rab);
       }
     }
   if (dielectric != 0) {
@@ -575,6 +589,23 @@ extern "C" int rism3d_solvent_chi_internal(const rism3d_problem* pd, int m,
         ct[q] = closure(pd->closure, pd->beta * R.vr[q], t) + t;
       }
   for (int i = 0; i < m * m; ++i) R.ft(&ct[(size_t)i * n], &h[(size_t)i * n], R.dr * R.dr * R.dr, tmp);
-  for (size_t q = 0; q < N; ++q) chi_out[q] = R.wk[q] + pd->rho * h[q];
+  // chi_ij = w_ij + rho_i h_ij
+  for (int i = 0; i < m; ++i)
+    for (size_t q = (size_t)i * m * n; q < (size_t)(i + 1) * m * n; ++q)
+      chi_out[q] = R.wk[q] + R.rs[i] * h[q];
   return RISM3D_OK;
+}
+}  // namespace
+
+extern "C" int rism3d_solvent_chi_internal(const rism3d_problem* pd, int m,
+                                           const rism3d_site* solvent, double dielectric,
+                                           double* chi_out) {
+  return solvent_chi(pd, m, solvent, dielectric, nullptr, nullptr, chi_out);
+}
+
+extern "C" int rism3d_solvent_chi_mixture(const rism3d_problem* pd, int m,
+                                          const rism3d_site* solvent, const int* molecule,
+                                          const double* rho_sites, double* chi_out) {
+  if (!molecule) return RISM3D_BAD_ARG;
+  return solvent_chi(pd, m, solvent, 0.0, molecule, rho_sites, chi_out);
 }

@@ -1456,6 +1456,10 @@ int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
   for (int d = 0; d < 3; ++d) { pb.N[d] = pd->N[d]; pb.L[d] = pd->L[d]; pb.h[d] = pd->L[d] / pd->N[d]; }
   GridArgs& g = pb.g;
   std::memset(&g, 0, sizeof g);
+  for (int i = 0; i < MAX_SITES; ++i) {
+    g.sig[i] = 1.0;
+    g.rw[i] = 1.0;
+  }
   g.N0 = pd->N[0]; g.N1 = pd->N[1]; g.N2 = pd->N[2];
   g.NX = pd->N[0] / 2 + 1;
   g.NH = (g.NX + 15) / 16 * 16;
@@ -1632,6 +1636,15 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   for (int d = 0; d < 3; ++d) { pb.N[d] = pd.N[d]; pb.L[d] = pd.L[d]; pb.h[d] = pd.L[d] / pd.N[d]; }
   GridArgs& g = pb.g;
   std::memset(&g, 0, sizeof g);
+  for (int i = 0; i < MAX_SITES; ++i) {
+    g.sig[i] = 1.0;
+    g.rw[i] = 1.0;
+  }
+  if (in.opt.site_rho)
+    for (int i = 0; i < in.m; ++i) {
+      g.rw[i] = in.opt.site_rho[i] / in.pd->rho;
+      g.sig[i] = std::sqrt(g.rw[i]);
+    }
   g.N0 = pd.N[0]; g.N1 = pd.N[1]; g.N2 = pd.N[2];
   g.NX = pd.N[0] / 2 + 1;
   g.NH = (g.NX + 15) / 16 * 16;
@@ -1683,8 +1696,9 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   for (int i = 0; i < m; ++i)
     for (int j = 0; j <= i; ++j)
       for (int k = 0; k < nrad; ++k)
-        chi_tab[((size_t)i * (i + 1) / 2 + j) * nrad + k] =
-            in.chi_k[((size_t)i * m + j) * nrad + k] - (i == j ? 1.0 : 0.0);
+        chi_tab[((size_t)i * (i + 1) / 2 + j) * nrad + k] =  // S_ij = chi_ij sig_j / sig_i
+            in.chi_k[((size_t)i * m + j) * nrad + k] * (g.sig[j] / g.sig[i]) -
+            (i == j ? 1.0 : 0.0);
   for (int i = 0; i < m; ++i)
     for (int k = 0; k < nrad; ++k) {
       const double kk = (2 * k + 1) * g.dk_tab / 2;
@@ -1986,6 +2000,19 @@ static int solve_impl(const rism3d_problem* pd, int m, const rism3d_site* solven
   if (opt) in.opt = *opt; else rism3d_options_default(&in.opt);
   std::vector<double> chi_own;
   rism3d_problem pd_up;
+  if (in.opt.site_rho) {
+    // a mixture brings its own chi (rism3d_solvent_chi_mixture); the
+    // gradients are built for one density
+    bool bad = !chi_k || in.opt.chi3d;
+    for (int i = 0; i < m && !bad; ++i) bad = !(in.opt.site_rho[i] > 0);
+    int st = bad ? RISM3D_BAD_ARG : RISM3D_OK;
+    for (int b = 0; b < B && st == RISM3D_OK; ++b)
+      if (res[b].gradient || res[b].dmu_dsite || res[b].response) st = RISM3D_UNSUPPORTED;
+    if (st != RISM3D_OK) {
+      for (int b = 0; b < B; ++b) res[b].status = st;
+      return st;
+    }
+  }
   if (!chi_k && !in.opt.chi3d) {
     // chi from 1D DRISM on the upscaled radial grid (hnc3d.c:772-783,
     // 1471-1495): rmax = 4 max(L)/2, nrad = 16 max(N).

@@ -65,7 +65,8 @@ class Options(ctypes.Structure):
                 ("diis_step", ctypes.c_double), ("verbose", ctypes.c_int),
                 ("device", ctypes.c_int), ("dielectric", ctypes.c_double),
                 ("continuation", ctypes.c_int),
-                ("chi3d", ctypes.POINTER(ctypes.c_double))]
+                ("chi3d", ctypes.POINTER(ctypes.c_double)),
+                ("site_rho", ctypes.POINTER(ctypes.c_double))]
 
 
 class Result(ctypes.Structure):
@@ -99,7 +100,8 @@ class Info(ctypes.Structure):
 EXPORTS = ("rism3d_options_default", "rism3d_solute_solve", "rism3d_batch_solve",
            "rism3d_residual", "rism3d_jacobian", "rism3d_solvent_solve",
            "rism3d_restart_destroy",
-           "rism3d_solvent_chi", "rism3d_version", "rism3d_profile_start",
+           "rism3d_solvent_chi", "rism3d_solvent_chi_mixture", "rism3d_version",
+           "rism3d_profile_start",
            "rism3d_profile_read")
 
 KERNELS = ("closure_r2c_x", "fft_y_fwd", "zstar", "fft_y_inv", "c2r_x_resid")
@@ -139,6 +141,9 @@ def load():
         lib.rism3d_restart_destroy.restype = None
         lib.rism3d_solvent_chi.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_double, dp]
         lib.rism3d_solvent_chi.restype = ctypes.c_int
+        lib.rism3d_solvent_chi_mixture.argtypes = [P(Problem), ctypes.c_int, P(Site),
+                                                   P(ctypes.c_int), dp, dp]
+        lib.rism3d_solvent_chi_mixture.restype = ctypes.c_int
         lib.rism3d_version.argtypes = []
         lib.rism3d_version.restype = ctypes.c_char_p
         lib.rism3d_profile_start.argtypes = [ctypes.c_int]
@@ -233,7 +238,7 @@ def _dptr(a):
 
 def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=False,
                      derivatives=False, adjoint=False, response=False, chi3d=None,
-                     info=False):
+                     info=False, site_rho=None):
     """Solve the 3D-RISM solute problem on the GPU.
 
     Returns a dict like the reference alist: 'free-energy',
@@ -251,7 +256,9 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
     complex or its real part), in place of chi (no tau renormalisation).
     info=True adds 'info', the numbers print_info prints
     (bgy3d-potential.c:341-443; see info_dict), and 'medium', the solvent
-    potential U_v on the grid (the field of the reference's Context)."""
+    potential U_v on the grid (the field of the reference's Context).
+    site_rho: per-site densities of a solvent mixture, with chi from
+    rism_solvent_mixture."""
     lib = load()
     pd = problem_data(settings)
     opt = make_options(settings)
@@ -278,6 +285,12 @@ def hnc3d_run_solute(solute, solvent, settings, chi=None, restart=None, want_t=F
     if adjoint:
         dsite = np.full((len(su), 6), np.nan)
         res.dmu_dsite = _dptr(dsite)
+    rho_arr = None
+    if site_rho is not None:
+        rho_arr = np.ascontiguousarray(site_rho, dtype=np.float64)
+        if rho_arr.shape != (m,):
+            raise ValueError(f"site_rho must have {m} entries")
+        opt.site_rho = _dptr(rho_arr)
     inf = None
     if info:
         inf = Info()
@@ -474,6 +487,28 @@ def rism_solvent(solvent, settings):
     st = lib.rism3d_solvent_chi(ctypes.byref(pd), m, sv, eps, _dptr(chi))
     if st != OK:
         raise RismError(st, "rism3d_solvent_chi")
+    return {"susceptibility": chi}
+
+
+def rism_solvent_mixture(solvent, molecule, rho, settings):
+    """1D XRISM susceptibility of a solvent mixture (no counterpart in the
+    reference): `molecule[i]` the molecule of site i, `rho[i]` its number
+    density.  Returns {'susceptibility': chi} with
+    chi_ij = w_ij + rho_i h_ij, for hnc3d_run_solute(chi=, site_rho=rho)."""
+    lib = load()
+    pd = problem_data(settings)
+    sv = make_sites(solvent)
+    m = len(sv)
+    mol = np.ascontiguousarray(molecule, dtype=np.int32)
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    if mol.shape != (m,) or rho.shape != (m,):
+        raise ValueError(f"molecule and rho must have {m} entries")
+    chi = np.empty((m, m, pd.nrad))
+    st = lib.rism3d_solvent_chi_mixture(ctypes.byref(pd), m, sv,
+                                        mol.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                        _dptr(rho), _dptr(chi))
+    if st != OK:
+        raise RismError(st, "rism3d_solvent_chi_mixture")
     return {"susceptibility": chi}
 
 

@@ -139,6 +139,15 @@ typedef struct rism3d_options {
                           i(i+1)/2 + j, j <= i (solvent_kernel_file,
                           hnc3d.c:1434-1437, 1572-1590).  Replaces chi_k;
                           no tau renormalisation, as in the reference      */
+  const double *site_rho; /* NULL (every solvent site at pd->rho, as the
+                          reference, hnc3d.c:90-92), or double[m]: the
+                          number density of each solvent site of a
+                          mixture (no counterpart in the reference).  chi_k
+                          is then rism3d_solvent_chi_mixture's
+                          chi_ij = w_ij + rho_i h_ij, contracted as
+                          h_i = sum_j c_j chi_ji, tau_i = sum_j q_j chi_ji,
+                          and mu / the PMV factor weight site i by rho_i.
+                          Requires chi_k; no gradients (UNSUPPORTED)       */
 } rism3d_options;
 
 typedef struct rism3d_result {
@@ -285,6 +294,20 @@ int rism3d_solvent_solve(const rism3d_problem *pd,
 
 /* Replaces bgy3d_restart_destroy (bgy3d-impure.c:593-600). */
 void rism3d_restart_destroy(rism3d_restart *restart);
+
+/* 1D XRISM susceptibility of a solvent mixture (no counterpart in the
+   reference, whose solvent is one molecule at one density): molecule[i] the
+   molecule of site i (intramolecular w_ij only within a molecule),
+   rho_sites[i] its number density [A^-3] (equal within a molecule).  Per k,
+   H = (1 - W C D)^-1 W C W with D = diag(rho_sites), closure and
+   long-range handling as rism3d_solvent_chi, no dielectric correction;
+   pd->rho is unused.  chi_out double[m][m][nrad],
+   chi_ij = w_ij + rho_i h_ij (not symmetric when densities differ): the
+   chi_k of rism3d_solute_solve with options.site_rho = rho_sites. */
+int rism3d_solvent_chi_mixture(const rism3d_problem *pd,
+                               int m, const rism3d_site *solvent,
+                               const int *molecule, const double *rho_sites,
+                               double *chi_out);
 
 /* 1D DRISM solvent susceptibility, replaces rism_solvent
    (rism.f90:57-102) with t_buf == NULL: chi_out is double[m][m][nrad].

@@ -145,11 +145,16 @@ def force_field_long(solute, solvent, r):
     return v
 
 
-def omega_fourier(sites, k):
+def omega_fourier(sites, k, molecule=None):
+    """w_ij(k) = j0(k r_ij); with `molecule` (a molecule index per site)
+    zero between sites of different molecules (the intramolecular matrix of
+    a mixture)."""
     m = len(sites)
     w = np.zeros((m, m, k.size))
     for j in range(m):
         for i in range(m):
+            if molecule is not None and molecule[i] != molecule[j]:
+                continue
             rab = np.linalg.norm(_xyz(sites[i]) - _xyz(sites[j]))
             w[i, j] = j0(k * rab)
     return w
@@ -261,7 +266,7 @@ def oz_vv_equation_c_t(rho, c, w):
     ck = np.moveaxis(c, -1, 0)
     wk = np.moveaxis(w, -1, 0)
     H = wk @ ck
-    T = np.eye(m)[None] - H * rho
+    T = np.eye(m)[None] - H * rho  # rho: scalar, or per site (scales column j)
     H = H @ wk
     H = np.linalg.solve(T, H)
     return np.moveaxis(H - ck, 0, -1)
@@ -281,7 +286,8 @@ def oz_vv_equation_c_t1(rho, c, w, dc):
     dck = np.moveaxis(dc, -1, 0)
     A = np.eye(m)[None] - (wk @ ck) * rho
     H = np.linalg.solve(A, wk @ ck @ wk)
-    dH = np.linalg.solve(A, wk @ dck @ (wk + rho * H))
+    rho_rows = np.asarray(rho)[..., None] if np.ndim(rho) else rho  # (rho H)_qj = rho_q H_qj
+    dH = np.linalg.solve(A, wk @ dck @ (wk + rho_rows * H))
     return np.moveaxis(dH - dck, 0, -1)
 
 
@@ -397,3 +403,54 @@ def rism_solvent_renorm(solvent, rmax, nrad, chi, alpha=ALPHA):
     q = np.array([s[4] for s in solvent], dtype=np.float64)
     v = coulomb_long_fourier(k, alpha)
     return np.einsum("ijk,j->ik", chi, q) * v[None, :]
+
+def rism_vv_mixture(method, nrad, rmax, beta, sites, molecule, rho_sites, atol=1e-12,
+                    verbose=False):
+    """XRISM of a mixture (no counterpart in the reference, whose solvent is
+    one molecule at one density, hnc3d.c:90-92): sites of several molecules,
+    `molecule[i]` the molecule of site i, `rho_sites[i]` its number density.
+    Per k, H = (1 - W C D)^-1 W C W with D = diag(rho), W block-diagonal by
+    molecule; the closure and long-range handling as rism_vv (no DRISM).
+    Returns dict(t, chi) with chi_ij = w_ij + rho_i h_ij, the matrix whose
+    transpose contracts the solute's c: h_i = sum_j c_j chi_ji."""
+    m = len(sites)
+    rho = np.asarray(rho_sites, dtype=np.float64)
+    if m == 1:
+        rho = float(rho[0])
+    r, dr, k, dk = mkgrid(rmax, nrad)
+    vr, vk = force_field(sites, sites, r, k)
+    wk = omega_fourier(sites, k, molecule)
+    FT_BW = (2 * PI) ** 3
+
+    def iterate_t(t):
+        c = C.closure(method, beta, vr, t)
+        c = fourier_rows(c) * dr ** 3 - beta * vk
+        dt = oz_vv_equation_c_t(rho, c, wk) - beta * vk
+        return fourier_rows(dt) * (dk ** 3 / FT_BW) - t
+
+    def jacobian_rt(rt, drt):
+        t = rt / r
+        dt = drt / r
+        c = fourier_rows(C.closure(method, beta, vr, t)) * dr ** 3 - beta * vk
+        dc = fourier_rows(C.closure1(method, beta, vr, t, dt)) * dr ** 3
+        ddt = fourier_rows(oz_vv_equation_c_t1(rho, c, wk, dc)) * (dk ** 3 / FT_BW)
+        return r * (ddt - dt)
+
+    rt, stats = newton_krylov(lambda x: r * iterate_t(x / r), np.zeros((m, m, nrad)),
+                              J=jacobian_rt if method != C.PY else None, atol=atol,
+                              verbose=verbose)
+    t = rt / r
+    c = C.closure(method, beta, vr, t)
+    h = fourier_rows(c + t) * dr ** 3
+    chi = wk + np.reshape(rho, (-1, 1, 1)) * h
+    return {"t": t, "chi": chi, "stats": stats}
+
+
+def renorm_mixture(solvent, rmax, nrad, chi, alpha=ALPHA):
+    """tau_i(k) = sum_j q_j chi_ji(k) 4pi e^{-k^2/4a^2}/k^2: the long-range
+    response of site i to the solute field through every site j, the
+    transposed contraction of rism_solvent_renorm (the same for a symmetric
+    chi)."""
+    _r, _dr, k, _dk = mkgrid(rmax, nrad)
+    q = np.array([s[4] for s in solvent], dtype=np.float64)
+    return np.einsum("jik,j->ik", chi, q) * coulomb_long_fourier(k, alpha)[None, :]

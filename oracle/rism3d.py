@@ -250,17 +250,35 @@ def irfft(pd, a):
 class Solute3D:
     """Everything hnc3d_solute_solve sets up before SNESSolve."""
 
-    def __init__(self, pd, solvent, solute, chi_1d, renorm=True, chi3d=None):
+    def __init__(self, pd, solvent, solute, chi_1d, renorm=True, chi3d=None, site_rho=None):
         """chi3d: None, or chi - 1 of a 3D solvent solve on this grid,
         [m(m+1)/2][N2][N1][N0/2+1] by pair i(i+1)/2 + j (j <= i): the
         --solvent-3d path (solvent_kernel_file, hnc3d.c:1434-1437 and
-        1572-1590), which has no tau renormalisation."""
+        1572-1590), which has no tau renormalisation.
+        site_rho: None (every site at pd.rho, as the reference), or the
+        number density of each solvent site of a mixture (no counterpart in
+        the reference): chi_1d is then the non-symmetric
+        chi_ij = w_ij + rho_i h_ij of rism1d.rism_vv_mixture, the solute's
+        c is contracted as h_i = sum_j c_j chi_ji, tau_i = sum_j q_j chi_ji
+        (rism1d.renorm_mixture), and mu and the PMV factor weight site i
+        by rho_i."""
         self.pd = pd
         self.solvent = solvent
         self.solute = solute
         self.m = len(solvent)
         self.q = np.array([s[4] for s in solvent])
-        if chi3d is not None:
+        self.rho_sites = (np.full(self.m, pd.rho) if site_rho is None
+                          else np.asarray(site_rho, dtype=np.float64))
+        if site_rho is not None:
+            m = self.m
+            dk = math.pi / pd.rmax
+            s_kv = rism1d.renorm_mixture(solvent, pd.rmax, pd.nrad, chi_1d)
+            tau = np.stack([ktab(pd, s_kv[i], dk) for i in range(m)]).astype(np.complex128)
+            # chi[i, j] holds chi_ji - delta_ij: star() then contracts as the
+            # transposed matrix
+            self.chi = np.stack([np.stack([ktab(pd, chi_1d[j, i] - (1.0 if i == j else 0.0), dk)
+                                           for j in range(m)]) for i in range(m)])
+        elif chi3d is not None:
             m = self.m
             self.chi = np.empty((m, m) + chi3d.shape[1:])
             for i in range(m):
@@ -365,11 +383,13 @@ class Solute3D:
         cl = -beta * self.q[:, None, None, None] * uc[None]
         x = -beta * self.v + t
 
-        def mu_of(method):
-            dens = C.form(method, x, h, c, cl).sum(axis=0)
-            return pd.rho * math.fsum(dens.ravel()) * pd.h3 / beta
+        rs = self.rho_sites
 
-        kc = sum(math.fsum(c[i].ravel()) for i in range(self.m)) * pd.rho * pd.h3
+        def mu_of(method):
+            dens = C.form(method, x, h, c, cl)
+            return math.fsum(rs[i] * math.fsum(dens[i].ravel()) for i in range(self.m)) * pd.h3 / beta
+
+        kc = sum(rs[i] * math.fsum(c[i].ravel()) for i in range(self.m)) * pd.h3
         return {
             "mu": mu_of(pd.closure),
             "mu_hnc": mu_of(C.HNC), "mu_kh": mu_of(C.KH), "mu_gf": mu_of(C.PY),

@@ -165,3 +165,18 @@ def test_petsc_vec_format(tmp_path):
     assert raw[:4] == (1211214).to_bytes(4, "big") and raw[4:8] == (24).to_bytes(4, "big")
     assert np.frombuffer(raw[8:], dtype=">f8")[5] == 2.5
     assert np.array_equal(petsc_io.read_vec(path, a.shape), a)
+
+
+def test_mixture_rejects_gradients():
+    """options.site_rho: the gradients are built for one density, and the
+    mixture's chi must come from the caller."""
+    R = _R()
+    W = "water, PR-SPC/E"
+    base = {"L": 10, "N": 32, "rmax": 40.0, "nrad": 1536}
+    rho = [0.0333295] * 3
+    with pytest.raises(R.RismError) as e:
+        R.hnc3d_run_solute(W, W, base, chi=_chi(), site_rho=rho, adjoint=True)
+    assert e.value.status == R.UNSUPPORTED
+    with pytest.raises(R.RismError) as e:
+        R.hnc3d_run_solute(W, W, base, site_rho=rho)
+    assert e.value.status == R.BAD_ARG
