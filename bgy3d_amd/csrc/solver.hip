@@ -163,13 +163,83 @@ void launch_zstar(int N2, int NX, int N1, int B, hipStream_t s, const SoluteDesc
 }
 
 // ------------------------------------------------------ device buffers
+// Device memory is cached across solves: a buffer released by one solve is
+// kept, keyed by device and byte size, and handed to the next request of
+// that size, so a batch of the same shape allocates nothing after the
+// first (hipMalloc / hipFree cost milliseconds per large buffer and
+// synchronise the device).  Every solve runs on one stream and synchronises
+// it before returning, so a released buffer is idle when handed out again.
+// RISM3D_CACHE_MB caps the cache (default 131072 MB, 0 disables it);
+// rism3d_cache_release() empties it.  On an allocation failure the cache of
+// that device is emptied and the allocation retried.
+struct MemCache {
+  std::mutex mu;
+  std::multimap<std::pair<int, size_t>, void*> blocks;
+  size_t bytes = 0;
+  size_t limit = (size_t)std::max(0, env_int("RISM3D_CACHE_MB", 131072)) << 20;
+};
+MemCache& mem_cache() {
+  static MemCache c;
+  return c;
+}
+void cache_empty(int dev) {  // dev < 0: every device
+  MemCache& c = mem_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (auto it = c.blocks.begin(); it != c.blocks.end();) {
+    if (dev >= 0 && it->first.first != dev) { ++it; continue; }
+    (void)hipSetDevice(it->first.first);
+    (void)hipFree(it->second);
+    c.bytes -= it->first.second;
+    it = c.blocks.erase(it);
+  }
+  (void)hipSetDevice(cur);
+}
+void* dev_alloc(size_t bytes) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  MemCache& c = mem_cache();
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.blocks.find({dev, bytes});
+    if (it != c.blocks.end()) {
+      void* p = it->second;
+      c.blocks.erase(it);
+      c.bytes -= bytes;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    cache_empty(dev);
+    HIPCHK(hipMalloc(&p, bytes));
+  }
+  return p;
+}
+void dev_free(void* p, size_t bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  MemCache& c = mem_cache();
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (c.bytes + bytes <= c.limit) {
+      c.blocks.insert({{dev, bytes}, p});
+      c.bytes += bytes;
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
 template <typename X>
 struct DevBuf {
   X* p = nullptr;
   size_t n = 0;
   DevBuf() = default;
   explicit DevBuf(size_t count) : n(count) {
-    if (count) HIPCHK(hipMalloc(&p, count * sizeof(X)));
+    if (count) p = static_cast<X*>(dev_alloc(count * sizeof(X)));
   }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -180,9 +250,52 @@ struct DevBuf {
   }
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p, n * sizeof(X));
     p = nullptr;
     n = 0;
+  }
+};
+
+// Pinned host staging for the small per-iteration transfers (solute
+// descriptors up, residual norms and Gram rows down): pageable copies are
+// staged by the runtime and block the host.  A ring of slots, each reused
+// only after the copy that last read it has completed.
+struct PinnedRing {
+  static constexpr int SLOTS = 4;
+  void* host[SLOTS] = {};
+  hipEvent_t ev[SLOTS] = {};
+  bool used[SLOTS] = {};
+  size_t bytes = 0;
+  int next = 0;
+  ~PinnedRing() {
+    for (int k = 0; k < SLOTS; ++k) {
+      if (host[k]) (void)hipHostFree(host[k]);
+      if (ev[k]) (void)hipEventDestroy(ev[k]);
+    }
+  }
+  void* slot(size_t need, int* k_out) {
+    if (need > bytes) {
+      for (int k = 0; k < SLOTS; ++k) {
+        if (used[k]) HIPCHK(hipEventSynchronize(ev[k]));
+        if (host[k]) HIPCHK(hipHostFree(host[k]));
+        host[k] = nullptr;
+        used[k] = false;
+      }
+      for (int k = 0; k < SLOTS; ++k) {
+        HIPCHK(hipHostMalloc(&host[k], need, hipHostMallocDefault));
+        if (!ev[k]) HIPCHK(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+      }
+      bytes = need;
+    }
+    const int k = next;
+    next = (next + 1) % SLOTS;
+    if (used[k]) HIPCHK(hipEventSynchronize(ev[k]));
+    *k_out = k;
+    return host[k];
+  }
+  void mark(int k, hipStream_t s) {
+    HIPCHK(hipEventRecord(ev[k], s));
+    used[k] = true;
   }
 };
 
@@ -324,7 +437,29 @@ struct Problem {
   DevBuf<int> d_cmask;
   int nparts = 0;
   std::vector<SoluteState> st;
+  PinnedRing up;    // descriptor uploads
+  PinnedRing down;  // norms and Gram rows
 };
+
+// Descriptors to the device through pinned staging (stream-ordered).
+void upload_sd(Problem& pb, const std::vector<SoluteDesc>& sd) {
+  const size_t bytes = sd.size() * sizeof(SoluteDesc);
+  int k = 0;
+  void* h = pb.up.slot(bytes, &k);
+  std::memcpy(h, sd.data(), bytes);
+  HIPCHK(hipMemcpyAsync(pb.d_sd.p, h, bytes, hipMemcpyHostToDevice, pb.s));
+  pb.up.mark(k, pb.s);
+}
+// pb.d_out to out[], synchronising the stream.
+void download_out(Problem& pb, std::vector<double>& out) {
+  const size_t bytes = out.size() * sizeof(double);
+  int k = 0;
+  void* h = pb.down.slot(bytes, &k);
+  HIPCHK(hipMemcpyAsync(h, pb.d_out.p, bytes, hipMemcpyDeviceToHost, pb.s));
+  pb.down.mark(k, pb.s);
+  HIPCHK(hipStreamSynchronize(pb.s));
+  std::memcpy(out.data(), h, bytes);
+}
 
 // Per-kernel HIP-event timing of the residual chain (rism3d_profile_*).
 // Kernel ids: 0 closure_r2c_x, 1 fft_y (forward), 2 zstar, 3 fft_y
@@ -354,7 +489,7 @@ int hist_of(const std::vector<SoluteDesc>& sd) {
 
 void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
   const GridArgs& g = pb.g;
-  HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), sd.size() * sizeof(SoluteDesc), hipMemcpyHostToDevice, pb.s));
+  upload_sd(pb, sd);
   const dim3 gx(resid_grid_x(g), g.m, pb.B);
   Profile& pf = g_prof;
   auto mark = [&](int k, int which) {
@@ -474,8 +609,7 @@ void rebase(Problem& pb, const Config& cf, const std::vector<int>& mask, bool ad
   }
   evaluate(pb, false, sd);
   std::vector<double> out((size_t)B * NPART);
-  HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, pb.s));
-  HIPCHK(hipStreamSynchronize(pb.s));
+  download_out(pb, out);
   HIPCHK(hipGetLastError());
   rinf.assign(B, 0.0);
   r2.assign(B, 0.0);
@@ -675,8 +809,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
       d.r16 = fp32 && pb.r16;
     }
     evaluate(pb, fp32, sd);
-    HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, pb.s));
-    HIPCHK(hipStreamSynchronize(pb.s));
+    download_out(pb, out);
     HIPCHK(hipGetLastError());
 
     std::vector<int> rebase_mask(B, 0);
@@ -804,8 +937,7 @@ struct OneSolute {
   // evaluate the prepared descriptor; returns (max|r|, ||r||^2)
   std::pair<double, double> run() {
     evaluate(pb, false, sd);
-    HIPCHK(hipMemcpyAsync(out.data(), pb.d_out.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, pb.s));
-    HIPCHK(hipStreamSynchronize(pb.s));
+    download_out(pb, out);
     HIPCHK(hipGetLastError());
     pb.st[b].iters++;
     return {out[(size_t)b * NPART], out[(size_t)b * NPART + 1]};
@@ -1482,7 +1614,10 @@ int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
   for (int i = 0; i < m; ++i) g.q[i] = solvent[i].charge;
   pb.mP = (int64_t)np * g.P;
   HIPCHK(hipStreamCreateWithFlags(&pb.s, hipStreamNonBlocking));
-  struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{pb.s};
+  struct StreamGuard {  // idle before its buffers return to the cache
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+  } sg{pb.s};
   make_twiddles(pb.tw, pb.N, pb.s);
 
   // omega_ij(k) in the chi3 slots of the pairs
@@ -1669,7 +1804,10 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   pb.mP = (int64_t)in.m * g.P;
 
   HIPCHK(hipStreamCreateWithFlags(&pb.s, hipStreamNonBlocking));
-  struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{pb.s};
+  struct StreamGuard {  // idle before its buffers return to the cache
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+  } sg{pb.s};
   make_twiddles(pb.tw, pb.N, pb.s);
 
   // Solvent kernel: chi - delta and the renormalisation tables
@@ -2148,6 +2286,12 @@ int rism3d_solvent_solve(const rism3d_problem* pd, int m, const rism3d_site* sol
 int rism3d_solvent_chi(const rism3d_problem* pd, int m, const rism3d_site* solvent,
                        double dielectric, double* chi_out) {
   return rism3d_solvent_chi_internal(pd, m, solvent, dielectric, chi_out);
+}
+
+int rism3d_cache_release(void) {
+  std::lock_guard<std::mutex> lock(rism::g_mutex);
+  rism::cache_empty(-1);
+  return RISM3D_OK;
 }
 
 const char* rism3d_version(void) { return "rism3d-mi355x 0.1 (gfx950)"; }

@@ -101,6 +101,7 @@ EXPORTS = ("rism3d_options_default", "rism3d_solute_solve", "rism3d_batch_solve"
            "rism3d_residual", "rism3d_jacobian", "rism3d_solvent_solve",
            "rism3d_restart_destroy",
            "rism3d_solvent_chi", "rism3d_solvent_chi_mixture", "rism3d_version",
+           "rism3d_cache_release",
            "rism3d_profile_start",
            "rism3d_profile_read")
 
@@ -510,6 +511,11 @@ def rism_solvent_mixture(solvent, molecule, rho, settings):
     if st != OK:
         raise RismError(st, "rism3d_solvent_chi_mixture")
     return {"susceptibility": chi}
+
+
+def cache_release():
+    """Free the device buffers the library keeps between solves."""
+    load().rism3d_cache_release()
 
 
 def profile_start(device=0):

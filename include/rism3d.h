@@ -328,6 +328,12 @@ int rism3d_solvent_chi(const rism3d_problem *pd,
 int rism3d_profile_start(int device);
 int rism3d_profile_read(double *ms, double *bytes, long *launches);
 
+/* Device memory is cached across solves (buffers of the same size are
+   reused instead of freed; RISM3D_CACHE_MB caps the cache, default
+   131072).  This frees every cached buffer on every device.  No
+   counterpart in the reference, which allocates and frees per solve. */
+int rism3d_cache_release(void);
+
 /* Library build/version string, for diagnostics. */
 const char *rism3d_version(void);
 
