@@ -241,13 +241,15 @@ fft_y(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     // all loads first (the row padding keeps every column of a tile in
     // bounds; columns past NX are zeroed after the load)
     constexpr int PTS = N1 * LINES, PT = (PTS + NT - 1) / NT;
+    const int nl = min(LINES, g.NX - kx0);
     cpx<T> v[PT];
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
       const int e = threadIdx.x + p * NT;
       if (PT * NT == PTS || e < PTS) {
         const int y = e / LINES, l = e - y * LINES;
-        v[p] = ldc(W + y * g.NH + l);
+        // the last tile (the Nyquist column) reads only its valid columns
+        if (nl == LINES || l < nl) v[p] = ldc(W + y * g.NH + l);
       }
     }
 #pragma unroll
@@ -618,7 +620,8 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
         const int e = tid + p * NT;
         if (PPT * NT == PTS || e < PTS) {
           const int z = e / LINES, l = e - z * LINES;
-          v[p] = ldc(Ws + z * zs + l);
+          // the last tile (the Nyquist column) reads only its valid columns
+          if (nl == LINES || l < nl) v[p] = ldc(Ws + z * zs + l);
         }
       }
 #pragma unroll
