@@ -439,6 +439,24 @@ struct Problem {
   std::vector<SoluteState> st;
   PinnedRing up;    // descriptor uploads
   PinnedRing down;  // norms and Gram rows
+  // The residual chain (five kernels + the norm reduction) as HIP graphs,
+  // one per working precision and history width, captured on first use and
+  // replayed every iteration (the descriptors they read are uploaded
+  // before each replay).  Re-captured if the grid arguments change.
+  // RISM3D_GRAPHS=0 launches the kernels one by one instead.
+  struct Graphs {
+    hipGraph_t graph[2][2] = {};
+    hipGraphExec_t exec[2][2] = {};
+    GridArgs key[2][2];
+    ~Graphs() {
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+          if (exec[a][b]) (void)hipGraphExecDestroy(exec[a][b]);
+          if (graph[a][b]) (void)hipGraphDestroy(graph[a][b]);
+        }
+    }
+  } graphs;
+  int use_graphs = env_int("RISM3D_GRAPHS", 1);
 };
 
 // Descriptors to the device through pinned staging (stream-ordered).
@@ -495,26 +513,47 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
   auto mark = [&](int k, int which) {
     if (pf.on) HIPCHK(hipEventRecord(pf.ev[k][which], pb.s));
   };
-  if (fp32) {
-    using T = float;
-    constexpr int LN = y_lines<T>();
-    const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
-    mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, hist_of(sd)); mark(0, 1);
-    mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(1, 1);
-    mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
-    mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(3, 1);
-    mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, hist_of(sd)); mark(4, 1);
+  const int hist = hist_of(sd);
+  auto chain = [&]() {
+    if (fp32) {
+      using T = float;
+      constexpr int LN = y_lines<T>();
+      const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
+      mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, hist); mark(0, 1);
+      mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(1, 1);
+      mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
+      mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(3, 1);
+      mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, hist); mark(4, 1);
+    } else {
+      using T = double;
+      constexpr int LN = y_lines<T>();
+      const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
+      mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist); mark(0, 1);
+      mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(1, 1);
+      mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p); mark(2, 1);
+      mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(3, 1);
+      mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist); mark(4, 1);
+    }
+    hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, pb.d_sd.p, pb.nparts, pb.d_out.p);
+  };
+  if (pb.use_graphs && !pf.on) {
+    auto& G = pb.graphs;
+    const int a = fp32 ? 1 : 0, b = hist > 8 ? 1 : 0;
+    if (!G.exec[a][b] || std::memcmp(&G.key[a][b], &g, sizeof g) != 0) {
+      if (G.exec[a][b]) HIPCHK(hipGraphExecDestroy(G.exec[a][b]));
+      if (G.graph[a][b]) HIPCHK(hipGraphDestroy(G.graph[a][b]));
+      G.exec[a][b] = nullptr;
+      G.graph[a][b] = nullptr;
+      HIPCHK(hipStreamBeginCapture(pb.s, hipStreamCaptureModeThreadLocal));
+      chain();
+      HIPCHK(hipStreamEndCapture(pb.s, &G.graph[a][b]));
+      HIPCHK(hipGraphInstantiate(&G.exec[a][b], G.graph[a][b], nullptr, nullptr, 0));
+      std::memcpy(&G.key[a][b], &g, sizeof g);
+    }
+    HIPCHK(hipGraphLaunch(G.exec[a][b], pb.s));
   } else {
-    using T = double;
-    constexpr int LN = y_lines<T>();
-    const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
-    mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist_of(sd)); mark(0, 1);
-    mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(1, 1);
-    mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p); mark(2, 1);
-    mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(3, 1);
-    mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist_of(sd)); mark(4, 1);
+    chain();
   }
-  hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, pb.d_sd.p, pb.nparts, pb.d_out.p);
   if (pf.on) {
     const double sc = fp32 ? 4.0 : 8.0;  // working-type scalar
     const double Kv = (double)g.NX * g.N1 * g.N2;  // spectral points without padding

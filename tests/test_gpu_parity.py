@@ -278,3 +278,34 @@ def test_continuation_hnc(chi_w):
     dmu = abs(m["free-energy"] - obs["mu"])
     print(f"  mixed: status {m['status']} iters {m['iterations']} |dmu| {dmu:.2e}", flush=True)
     assert m["status"] == "OK" and dmu < 0.01
+
+
+def test_graph_replay_matches_eager(chi_w):
+    """The residual chain replayed as HIP graphs (default) against the same
+    kernels launched one by one (RISM3D_GRAPHS=0): a mixed-precision batch
+    (FP32 and FP64 graphs, hundreds of replays with uploads, read-backs and
+    rebases between them) gives bit-identical mu and iteration counts."""
+    print("case graphs vs eager", flush=True)
+    import os
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd import synthetic as S
+    sols = [S.solute(s) for s in range(4)]
+    st = dict(README, N=32, L=8.0, rmax=40.0, nrad=1536, closure="KH",
+              **{"norm-tol": 1e-10, "precision": "mixed"})
+    old = os.environ.get("RISM3D_GRAPHS")
+    try:
+        os.environ["RISM3D_GRAPHS"] = "1"
+        a = R.batch_solve(sols, "water, PR-SPC/E", st, chi_w)
+        os.environ["RISM3D_GRAPHS"] = "0"
+        b = R.batch_solve(sols, "water, PR-SPC/E", st, chi_w)
+    finally:
+        if old is None:
+            os.environ.pop("RISM3D_GRAPHS", None)
+        else:
+            os.environ["RISM3D_GRAPHS"] = old
+    for x, y in zip(a, b):
+        print(f"  {x['status']} {x['free-energy']!r} {y['free-energy']!r} iters {x['iterations']} {y['iterations']}",
+              flush=True)
+        assert x["status"] == "OK" and y["status"] == "OK"
+        assert x["free-energy"] == y["free-energy"]
+        assert x["iterations"] == y["iterations"]
