@@ -612,9 +612,9 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
   for (int s = 0; s < MS; ++s) {
     cpx<T>* Ws = W + s * g.K;
     {
-      // all loads first; the row padding keeps every column of a tile in
-      // bounds, columns past NX are zeroed after the load
-      cpx<T> v[PPT];
+      // all loads first, staged in this site's registers of c (its FFT
+      // result lands there after the pass); columns past NX are zeroed
+      cpx<T>* v = c[s];
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
         const int e = tid + p * NT;
