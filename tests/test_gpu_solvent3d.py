@@ -239,3 +239,35 @@ def test_hnc3d_g1_regression_gpu(solvent96, name):
             assert near(d[a], mo["d"][a], 4), (i, a, d[a], mo["d"][a])
             for b in range(3):
                 assert near(Q[a][b], mo["Q"][a][b], 4), (i, a, b, Q[a][b], mo["Q"][a][b])
+
+
+def test_cli_solvent_3d(tmp_path, capsys, monkeypatch):
+    """`energy --hnc --solvent-3d` as test/Makefile runs it: the solvent run
+    writes g%d%d.bin and x%d%d-fft.bin, the solute run reads the chi back
+    and prints the print_info / print_chempot lines."""
+    print("case cli --solvent-3d n32", flush=True)
+    monkeypatch.chdir(tmp_path)
+    from bgy3d_amd import cli
+    from bgy3d_amd import petsc_io
+    from bgy3d_amd import rism3d as R
+    args = ["energy", "--hnc", "--N", "32", "--L", "8", "--beta", str(BETA), "--rho", str(RHO),
+            "--norm-tol", "1e-11", "--max-iter", "3000", "--closure", "KH",
+            "--solvent", "water", "--solvent-3d"]
+    assert cli.main(args) == 0
+    for name in ("g00.bin", "g10.bin", "g22.bin", "x00-fft.bin", "x21-fft.bin"):
+        assert (tmp_path / name).exists(), name
+    capsys.readouterr()
+    assert cli.main(args + ["--solute", "water"]) == 0
+    text = capsys.readouterr().out
+    print(text, flush=True)
+    mu = {ln.split("(")[1].split(")")[0]: float(ln.split("=")[1].split()[0])
+          for ln in text.splitlines() if ln.startswith(" # mu =")}
+    chi = [petsc_io.read_vec(f"x{i}{j}-fft.bin", (32, 32, 17, 2))[..., 0]
+           for i in range(3) for j in range(i + 1)]
+    ref = R.hnc3d_run_solute("water", "water", {"N": 32, "L": 8, "beta": BETA, "rho": RHO,
+                                                "norm-tol": 1e-11, "max-iter": 3000,
+                                                "closure": "KH"}, chi3d=chi)
+    assert abs(mu["KH"] - ref["free-energy-kh"]) < 1e-6
+    assert abs(mu["HNC"] - ref["free-energy-hnc"]) < 1e-6
+    assert "q_v = " in text and "<ρ_v|U_u> = " in text
+    assert (tmp_path / "g0.bin").exists() and (tmp_path / "g2.bin").exists()
