@@ -70,7 +70,7 @@ def measured_traffic(workload, kernel):
 
 
 def cpu_baseline(solute, L_full, chi, fp64_iters, n_eval=4):
-    """The oracle (numpy FP64, 1 core) on a bounded sample: setup of one
+    """The oracle (numpy FP64, FFTs threaded) on a bounded sample: setup of one
     solute plus n_eval residual evaluations at 128^3, scaled by the mean
     number of residual evaluations the FP64 GPU solves needed."""
     from oracle import closures as C
@@ -87,8 +87,9 @@ def cpu_baseline(solute, L_full, chi, fp64_iters, n_eval=4):
         t = t + 0.1 * r
     t_eval = (time.perf_counter() - t0) / n_eval
     per_solve = t_setup + fp64_iters * t_eval
-    return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle/rism3d.py (numpy FP64, single thread) on solute seed 0 at "
+    return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": O.FFT_WORKERS, "kind": "port",
+            "sample": (f"oracle/rism3d.py (numpy FP64; FFTs on {O.FFT_WORKERS} threads, the "
+                       f"rest on one) on solute seed 0 at "
                        f"128^3: setup {t_setup:.2f} s + {n_eval} residual evaluations at "
                        f"{t_eval:.2f} s each, scaled to {fp64_iters:.0f} evaluations "
                        f"(mean FP64 MDIIS count on the GPU)")}

@@ -33,8 +33,10 @@ reference's own reproducibility floor; grid invariants (translation by whole
 cells); in tests/test_oracle_*.py.
 """
 import math
+import os
 
 import numpy as np
+import scipy.fft
 from scipy.special import erf, erfc
 
 from . import closures as C
@@ -238,13 +240,19 @@ def solvent_kernel_rism(pd, solvent, chi_1d):
     return chi, tau
 
 
+# pocketfft through scipy.fft on up to 16 host threads (the same transforms
+# as numpy.fft, which runs them on one thread)
+FFT_WORKERS = max(1, min(16, os.cpu_count() or 1))
+
+
 def rfft(pd, a):
-    return np.fft.rfftn(a, axes=(-3, -2, -1))
+    return scipy.fft.rfftn(a, axes=(-3, -2, -1), workers=FFT_WORKERS)
 
 
 def irfft(pd, a):
     """Unnormalised c2r (FFTW semantics)."""
-    return np.fft.irfftn(a, s=pd.N[::-1], axes=(-3, -2, -1), norm="forward")
+    return scipy.fft.irfftn(a, s=pd.N[::-1], axes=(-3, -2, -1), norm="forward",
+                            workers=FFT_WORKERS)
 
 
 class Solute3D:
