@@ -1,6 +1,6 @@
-"""The larger benchmark configurations as GPU parity cases.
+"""Two large single-solute workloads as GPU cases.
 
-Config 2 (precision study): one 120-site synthetic solute, net charge -2,
+Precision study: one 120-site synthetic solute, net charge -2,
 PSE-3, --L 32 --N 256 (64 A box, h = 0.25 A), PR-SPC/E water.  |dmu| and
 max|dg| against the FP64 solve of: FP32 only (FP32 passes around the
 starting point until their floor; reported, no bar: the floor is the point
@@ -8,7 +8,7 @@ of the study), and the mixed path (defect correction, then 3 FP64 polish
 steps; bar |dmu| < 0.01 kcal/mol).  There is no published mu for this case
 (parity of the FP64 path to the oracle is covered on smaller grids).
 
-Config 3 (gradients): one 30-site solute, HNC, --L 20 --N 128; adjoint
+Gradients: one 30-site solute, HNC, --L 20 --N 128; adjoint
 d mu / d theta against central finite differences of the GPU mu (h = 1e-4)
 to 1e-4 relative, and the time of forward + adjoint against forward alone.
 """
@@ -28,8 +28,8 @@ def chi160():
     return load_chi("chi_prspce_hnc_eps78.4_rmax160_nrad4096.npy")
 
 
-def test_config2_precision_study(chi160):
-    print("case config2 256^3 PSE3 120 sites Q=-2", flush=True)
+def test_precision_study_256_pse3_120_sites(chi160):
+    print("case precision study 256^3 PSE3 120 sites Q=-2", flush=True)
     from bgy3d_amd import rism3d as R
     from bgy3d_amd import synthetic as S
     sol = S.solute(2, n=120, net=-2)
@@ -57,8 +57,8 @@ def test_config2_precision_study(chi160):
     print(f"  fp64: mu {f64['free-energy']:.6f} iters {f64['iterations']} {t64:.2f} s", flush=True)
 
 
-def test_config3_adjoint(chi160):
-    print("case config3 128^3 HNC 30 sites adjoint", flush=True)
+def test_adjoint_128_hnc_30_sites(chi160):
+    print("case adjoint 128^3 HNC 30 sites", flush=True)
     from bgy3d_amd import rism3d as R
     from bgy3d_amd import synthetic as S
     sol = list(S.solute(3, n=30))

@@ -7,8 +7,7 @@
 * Exact adjoint d mu / d(x, y, z, q, sigma, epsilon) vs the oracle's
   adjoint (itself checked against finite differences of mu on the CPU,
   tests/test_oracle_gradients.py) to 1e-6 relative, and one direct central
-  finite difference of the GPU mu (h = 1e-4) to 1e-4 relative, the bar of
-  the north star.
+  finite difference of the GPU mu (h = 1e-4) to 1e-4 relative.
 * Linear-response gradient (the reference's --response, hnc3d.c:1969-2134),
   computed on the GPU from one adjoint solve, vs the oracle's literal
   restatement with 3 n linear solves: 1e-6 relative, on both the tau and
