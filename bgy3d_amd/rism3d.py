@@ -145,6 +145,8 @@ def load():
         lib.rism3d_solvent_chi_mixture.argtypes = [P(Problem), ctypes.c_int, P(Site),
                                                    P(ctypes.c_int), dp, dp]
         lib.rism3d_solvent_chi_mixture.restype = ctypes.c_int
+        lib.rism3d_cache_release.argtypes = []
+        lib.rism3d_cache_release.restype = ctypes.c_int
         lib.rism3d_version.argtypes = []
         lib.rism3d_version.restype = ctypes.c_char_p
         lib.rism3d_profile_start.argtypes = [ctypes.c_int]

@@ -180,3 +180,9 @@ def test_mixture_rejects_gradients():
     with pytest.raises(R.RismError) as e:
         R.hnc3d_run_solute(W, W, base, site_rho=rho)
     assert e.value.status == R.BAD_ARG
+
+
+def test_cache_release_without_work():
+    """rism3d_cache_release with an empty cache (no device needed)."""
+    R = _R()
+    assert R.load().rism3d_cache_release() == R.OK
