@@ -590,22 +590,26 @@ constexpr int zreg_lines() {
   return L;
 }
 
-template <typename T, int N2, int LINES, int MS>
+template <typename T, int N2, int LINES, int MS, bool NYQ>
 __global__ void __launch_bounds__(NT)
 zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw,
           const T* __restrict__ chi3, const T* __restrict__ tau3) {
   constexpr int PTS = N2 * LINES;
   constexpr int PPT = (PTS + NT - 1) / NT;
-  __shared__ cpx<T> lds[PTS];  // interleaved: point z of column l at z * LINES + l
-  const int b = g.zorder ? blockIdx.z : blockIdx.x;
+  __shared__ cpx<T> lds[PTS];  // interleaved: point z of line l at z * LINES + l
+  const int b = (NYQ || g.zorder) ? blockIdx.z : blockIdx.x;
   const SoluteDesc& d = sd[b];
   if (!d.active) return;
-  const int y = g.zorder ? blockIdx.y : blockIdx.z;
-  const int kx0 = (g.zorder ? blockIdx.x : blockIdx.y) * LINES;
+  // NYQ: lines are the rows y0 + l of the Nyquist column kx = N0/2 (grid
+  // x = ceil(N1 / LINES), y = 1, z = solute); otherwise the kx columns
+  // kx0 + l of row y.
+  const int y = NYQ ? blockIdx.x * LINES : (g.zorder ? blockIdx.y : blockIdx.z);
+  const int kx0 = NYQ ? g.NX - 1 : (g.zorder ? blockIdx.x : blockIdx.y) * LINES;
   const int64_t zs = (int64_t)g.N1 * g.NH;  // z stride
+  const int64_t ls = NYQ ? g.NH : 1;        // line stride
   const int64_t base = (int64_t)y * g.NH + kx0;
   cpx<T>* W = static_cast<cpx<T>*>(d.W) + base;
-  const int nl = min(LINES, g.NX - kx0);  // valid kx columns of this tile
+  const int nl = NYQ ? min(LINES, g.N1 - y) : min(LINES, g.NX - kx0);  // valid lines
   const int tid = threadIdx.x;
   cpx<T> c[MS][PPT];
 #pragma unroll
@@ -613,15 +617,15 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
     cpx<T>* Ws = W + s * g.K;
     {
       // all loads first, staged in this site's registers of c (its FFT
-      // result lands there after the pass); columns past NX are zeroed
+      // result lands there after the pass); lines past nl are zeroed
       cpx<T>* v = c[s];
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
         const int e = tid + p * NT;
         if (PPT * NT == PTS || e < PTS) {
           const int z = e / LINES, l = e - z * LINES;
-          // the last tile (the Nyquist column) reads only its valid columns
-          if (nl == LINES || l < nl) v[p] = ldc(Ws + z * zs + l);
+          // a partial tile reads only its valid lines
+          if (nl == LINES || l < nl) v[p] = ldc(Ws + z * zs + l * ls);
         }
       }
 #pragma unroll
@@ -638,15 +642,13 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
       if (PPT * NT == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
         c[s][p] = lds[e];
-        if (d.mode == 3 && l < nl) stc(Ws + z * zs + l, c[s][p]);  // forward spectrum only
+        if (d.mode == 3 && l < nl) stc(Ws + z * zs + l * ls, c[s][p]);  // forward spectrum only
       }
     }
     __syncthreads();
   }
   if (d.mode == 3) return;
   const bool linear = d.mode != 0;  // chi convolution only
-  const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
-  const double ky = iy * g.dky;
   const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + base;
 #pragma unroll
   for (int p = 0; p < PPT; ++p) {
@@ -654,15 +656,17 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
     if (!(PPT * NT == PTS || e < PTS)) continue;
     const int z = e / LINES, l = e - z * LINES;
     if (l >= nl) continue;
-    const int64_t kk = base + z * zs + l;  // k-grid index of this point
+    const int64_t kk = base + z * zs + l * ls;  // k-grid index of this point
     cpx<T> fs = {T(0), T(0)};
-    if (!linear) fs = ldc(Fs + z * zs + l);
+    if (!linear) fs = ldc(Fs + z * zs + l * ls);
     cpx<T> uc = {T(0), T(0)};
     if (!g.renorm && !linear) {
       // u_c(k) = 4 pi eps0^-1 exp(-k^2/4G^2)/k^2 F(k) s(k), 0 at k = 0
       // (bgy3d-force.h:140-147, bgy3d-solutes.c:767-792).
       const int iz = (z <= N2 / 2) ? z : z - N2;
-      const double k2 = kmag2((kx0 + l) * g.dkx, ky, iz * g.dkz);
+      const int yl = NYQ ? y + l : y;
+      const int iy = (yl <= g.N1 / 2) ? yl : yl - g.N1;
+      const double k2 = kmag2((NYQ ? kx0 : kx0 + l) * g.dkx, iy * g.dky, iz * g.dkz);
       const double shape = (k2 == 0.0) ? 0.0
           : (4 * M_PI * EPSILON0INV) * exp(-k2 / (4 * G_COULOMB * G_COULOMB)) / k2;
       uc = cscale(fs, T(shape));
@@ -715,7 +719,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
       const int e = tid + p * NT;
       if (PPT * NT == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
-        if (l < nl) stc(Ws + z * zs + l, lds[e]);
+        if (l < nl) stc(Ws + z * zs + l * ls, lds[e]);
       }
     }
     __syncthreads();

@@ -127,12 +127,25 @@ template <typename T, int N2>
 void launch_zreg_n(int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
                    const cpx<T>* tw, const T* chi, const T* tau) {
   constexpr int L = zreg_lines<T, N2>();
-  const int tiles = (NX + L - 1) / L;
+  // In FP32 the Nyquist column kx = N0/2 gets a launch of its own (NYQ,
+  // LINES rows per workgroup) when the tiles of LINES columns end just
+  // before it, instead of a tile that holds it alone (one workgroup in five
+  // at N0 = 128): 1.19 -> 1.08 ms per pass at 64 x 128^3 (2, 4, 8 or 16
+  // rows per Nyquist workgroup alike).  In FP64, and in the y passes, the
+  // separate launch cost more than the tile it replaced.  RISM3D_NYQ=0
+  // keeps it in the last tile.
+  static const int nyq_on = env_int("RISM3D_NYQ", 1);
+  const bool split = nyq_on && sizeof(T) == 4 && NX > 1 && (NX - 1) % L == 0;
+  const int tiles = split ? (NX - 1) / L : (NX + L - 1) / L;
   const dim3 grid = g.zorder ? dim3(tiles, N1, B) : dim3(B, tiles, N1);
-  if (g.m == 3)
-    hipLaunchKernelGGL((zstar_reg<T, N2, L, 3>), grid, dim3(NT), 0, s, sd, g, tw, chi, tau);
-  else
-    hipLaunchKernelGGL((zstar_reg<T, N2, L, 5>), grid, dim3(NT), 0, s, sd, g, tw, chi, tau);
+  const dim3 gn((N1 + L - 1) / L, 1, B);
+  if (g.m == 3) {
+    hipLaunchKernelGGL((zstar_reg<T, N2, L, 3, false>), grid, dim3(NT), 0, s, sd, g, tw, chi, tau);
+    if (split) hipLaunchKernelGGL((zstar_reg<T, N2, L, 3, true>), gn, dim3(NT), 0, s, sd, g, tw, chi, tau);
+  } else {
+    hipLaunchKernelGGL((zstar_reg<T, N2, L, 5, false>), grid, dim3(NT), 0, s, sd, g, tw, chi, tau);
+    if (split) hipLaunchKernelGGL((zstar_reg<T, N2, L, 5, true>), gn, dim3(NT), 0, s, sd, g, tw, chi, tau);
+  }
 }
 // zstar_reg for the solute residual with 3 or 5 solvent sites (RISM3D_ZSTAR_LDS=1 keeps
 // the all-sites-in-LDS kernel), zstar otherwise.
@@ -481,14 +494,15 @@ void download_out(Problem& pb, std::vector<double>& out) {
 
 // Per-kernel HIP-event timing of the residual chain (rism3d_profile_*).
 // Kernel ids: 0 closure_r2c_x, 1 fft_y (forward), 2 zstar, 3 fft_y
-// (inverse), 4 c2r_x_resid.  Bytes are the algorithmic (compulsory) HBM
-// traffic of each launch: every array the kernel must read or write, once.
+// (inverse), 4 c2r_x_resid; slots 0-4 count the FP64 launches, 5-9 the
+// FP32 ones.  Bytes are the algorithmic (compulsory) HBM traffic of each
+// launch: every array the kernel must read or write, once.
 struct Profile {
   bool on = false;
   hipEvent_t ev[5][2] = {};
-  double ms[5] = {0, 0, 0, 0, 0};
-  double bytes[5] = {0, 0, 0, 0, 0};
-  long launches[5] = {0, 0, 0, 0, 0};
+  double ms[10] = {};
+  double bytes[10] = {};
+  long launches[10] = {};
 };
 Profile g_prof;
 
@@ -578,12 +592,13 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
                            na * (2 * mK * 2 * sc + Kv * 2 * sc) + tables,
                            na * 2 * mK * 2 * sc, b4};
     HIPCHK(hipEventSynchronize(pf.ev[4][1]));
+    const int slot = fp32 ? 5 : 0;
     for (int k = 0; k < 5; ++k) {
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, pf.ev[k][0], pf.ev[k][1]));
-      pf.ms[k] += ms;
-      pf.bytes[k] += per[k];
-      pf.launches[k] += 1;
+      pf.ms[slot + k] += ms;
+      pf.bytes[slot + k] += per[k];
+      pf.launches[slot + k] += 1;
     }
   }
 }
@@ -2338,9 +2353,10 @@ const char* rism3d_version(void) { return "rism3d-mi355x 0.1 (gfx950)"; }
 int rism3d_profile_start(int device) {
   if (hipSetDevice(device) != hipSuccess) return RISM3D_NO_DEVICE;
   auto& pf = rism::g_prof;
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < 5; ++k)
     for (int w = 0; w < 2; ++w)
       if (!pf.ev[k][w] && hipEventCreate(&pf.ev[k][w]) != hipSuccess) return RISM3D_NO_DEVICE;
+  for (int k = 0; k < 10; ++k) {
     pf.ms[k] = 0;
     pf.bytes[k] = 0;
     pf.launches[k] = 0;
@@ -2351,7 +2367,7 @@ int rism3d_profile_start(int device) {
 
 int rism3d_profile_read(double* ms, double* bytes, long* launches) {
   auto& pf = rism::g_prof;
-  for (int k = 0; k < 5; ++k) {
+  for (int k = 0; k < 10; ++k) {
     if (ms) ms[k] = pf.ms[k];
     if (bytes) bytes[k] = pf.bytes[k];
     if (launches) launches[k] = pf.launches[k];

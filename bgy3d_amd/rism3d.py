@@ -527,9 +527,16 @@ def profile_start(device=0):
 
 
 def profile_read():
-    """{kernel: (ms, algorithmic bytes, launches)} since profile_start."""
-    ms = np.zeros(5)
-    by = np.zeros(5)
-    n = (ctypes.c_long * 5)()
+    """{kernel<type>: (ms, algorithmic bytes, launches)} since profile_start,
+    one entry per residual kernel and working precision, e.g.
+    "closure_r2c_x<float>"; instantiations never launched are left out."""
+    ms = np.zeros(10)
+    by = np.zeros(10)
+    n = (ctypes.c_long * 10)()
     load().rism3d_profile_read(_dptr(ms), _dptr(by), n)
-    return {k: (float(ms[i]), float(by[i]), int(n[i])) for i, k in enumerate(KERNELS)}
+    out = {}
+    for slot, typ in ((0, "double"), (5, "float")):
+        for i, k in enumerate(KERNELS):
+            if n[slot + i]:
+                out[f"{k}<{typ}>"] = (float(ms[slot + i]), float(by[slot + i]), int(n[slot + i]))
+    return out
