@@ -747,6 +747,17 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   const int row0 = blockIdx.x * ROWS;
   const int vrows = min(ROWS, nrows - row0);
   const cpx<T>* W = static_cast<const cpx<T>*>(d.W) + site * g.K + (int64_t)row0 * g.NH;
+  const int64_t off = site * g.P + (int64_t)row0 * N0;
+  constexpr int VW = Vec16<T>::W;
+  // FP32 defect-correction passes over whole tiles: the streams this
+  // thread reads after the inverse FFT (t, F(t_b), the BF16 residual
+  // history) are loaded together with the spectrum, so those loads overlap
+  // the FFT instead of following it.  ITP vectors per thread.
+  constexpr int ITP = (ROWS * N0) / (NT * VW);
+  constexpr bool PREF = sizeof(T) == 4 && ITP * NT * VW == ROWS * N0 && ITP <= 2;
+  const bool pref = PREF && d.r16 != 0 && d.mode == 5 && vrows == ROWS;
+  Vec16<T> ptv[PREF ? ITP : 1], pfb[PREF ? ITP : 1];
+  BF4 prh[PREF ? ITP : 1][H];
   {
     // all loads first; rows past the last one read the last row's address
     // and are zeroed
@@ -759,6 +770,19 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
         const int row = e / (M + 1);
         const int k = e - row * (M + 1);
         v[p] = ldc(W + min(row, vrows - 1) * g.NH + k);
+      }
+    }
+    if constexpr (PREF) {
+      if (pref) {
+#pragma unroll
+        for (int it = 0; it < ITP; ++it) {
+          const int64_t e = off + threadIdx.x * VW + it * NT * VW;
+          ptv[it] = ld16(static_cast<const T*>(d.t) + e);
+          pfb[it] = ld16(static_cast<const T*>(d.fb) + e);
+#pragma unroll
+          for (int j = 0; j < H; ++j)
+            if (j < d.ndot) prh[it][j] = ldbf4(static_cast<const uint16_t*>(d.rh[j]) + e);
+        }
       }
     }
 #pragma unroll
@@ -781,7 +805,6 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   }
   __syncthreads();
   fft_lines<T, M, ROWS, NT, true>(zb, LSM, vrows, tw_half);
-  const int64_t off = site * g.P + (int64_t)row0 * N0;
   const T* __restrict__ t = static_cast<const T*>(d.t) + off;
   T* __restrict__ r = static_cast<T*>(d.r) + off;  // (BF16 slots: addressed in e below)
   T* __restrict__ un = d.unew ? static_cast<T*>(d.unew) + off : nullptr;  // null: no u
@@ -793,9 +816,8 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
 #pragma unroll
   for (int j = 0; j < H; ++j) dot[j] = 0.0;
   const int mode = d.mode;
-  constexpr int VW = Vec16<T>::W;
-  for (int e = threadIdx.x * VW; e < vrows * N0; e += NT * VW) {
-    const Vec16<T> tv = ld16(t + e);
+  // one vector of VW points at e: t, F(t_b) (mode 5) and the history rh
+  auto point = [&](int e, const Vec16<T>& tv, const Vec16<T>& f, const Vec16<T>* rh) {
     Vec16<T> z;  // t' at points e.. of row e / N0: z_n = t'_{2n} + i t'_{2n+1}
     {
       const cpx<T>* zrow = zb + (e / N0) * LSM;
@@ -807,24 +829,6 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
         z.v[2 * h + 1] = zz.im;
       }
     }
-    Vec16<T> f;  // F(t_b), defect correction
-    if (mode == 5) f = ld16(static_cast<const T*>(d.fb) + off + e);
-    Vec16<T> rh[H];
-#pragma unroll
-    for (int j = 0; j < H; ++j)
-      if (j < nd) {
-        if constexpr (sizeof(T) == 4) {
-          if (r16) {
-            const BF4 q = ldbf4(static_cast<const uint16_t*>(d.rh[j]) + off + e);
-            rh[j].v[0] = bf16_val(q.lo & 0xFFFFu);
-            rh[j].v[1] = bf16_val(q.lo >> 16);
-            rh[j].v[2] = bf16_val(q.hi & 0xFFFFu);
-            rh[j].v[3] = bf16_val(q.hi >> 16);
-            continue;
-          }
-        }
-        rh[j] = ld16(static_cast<const T*>(d.rh[j]) + off + e);
-      }
     Vec16<T> rv, uv;
     if (mode == 5) {  // defect correction: F(t_b) + K dc - delta
 #pragma unroll
@@ -872,6 +876,50 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
           for (int k = 0; k < VW; ++k) pj += rv.v[k] * rh[j].v[k];
           dot[j] += (double)pj;
         }
+    }
+  };
+  auto unbf = [](BF4 q) {
+    Vec16<T> x;
+    if constexpr (sizeof(T) == 4) {
+      x.v[0] = bf16_val(q.lo & 0xFFFFu);
+      x.v[1] = bf16_val(q.lo >> 16);
+      x.v[2] = bf16_val(q.hi & 0xFFFFu);
+      x.v[3] = bf16_val(q.hi >> 16);
+    }
+    return x;
+  };
+  bool done = false;
+  if constexpr (PREF) {
+    if (pref) {
+#pragma unroll
+      for (int it = 0; it < ITP; ++it) {
+        Vec16<T> rh[H];
+#pragma unroll
+        for (int j = 0; j < H; ++j)
+          if (j < nd) rh[j] = unbf(prh[it][j]);
+        point(threadIdx.x * VW + it * NT * VW, ptv[it], pfb[it], rh);
+      }
+      done = true;
+    }
+  }
+  if (!done) {
+    for (int e = threadIdx.x * VW; e < vrows * N0; e += NT * VW) {
+      const Vec16<T> tv = ld16(t + e);
+      Vec16<T> f;  // F(t_b), defect correction
+      if (mode == 5) f = ld16(static_cast<const T*>(d.fb) + off + e);
+      Vec16<T> rh[H];
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+        if (j < nd) {
+          if constexpr (sizeof(T) == 4) {
+            if (r16) {
+              rh[j] = unbf(ldbf4(static_cast<const uint16_t*>(d.rh[j]) + off + e));
+              continue;
+            }
+          }
+          rh[j] = ld16(static_cast<const T*>(d.rh[j]) + off + e);
+        }
+      point(e, tv, f, rh);
     }
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
