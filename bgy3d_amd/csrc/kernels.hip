@@ -130,6 +130,13 @@ __device__ __forceinline__ void stbf4(uint16_t* p, BF4 q) {
   *(G*)p = x;
 }
 
+// Solute of a residual-kernel workgroup: grid index idx of n, reversed when
+// bit `bit` of g.rev is set, so that a kernel can start on the solutes the
+// previous kernel of the chain wrote last.
+__device__ __forceinline__ int solute_of(const GridArgs& g, int bit, int idx, int n) {
+  return (g.rev & bit) ? n - 1 - idx : idx;
+}
+
 // ------------------------------------------- 1. closure fused with x r2c
 // t = sum_j coef_j u_j (the MDIIS step; written back as the iterate) or the
 // given t, then c = closure(beta v, t), then the x-axis r2c of c.
@@ -142,7 +149,7 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
   constexpr int ROWS = rows_for(M);
   constexpr int LSM = lstride_for(M);
   __shared__ cpx<T> lds[ROWS * LSM];
-  const SoluteDesc& d = sd[blockIdx.z];
+  const SoluteDesc& d = sd[solute_of(g, 1, blockIdx.z, gridDim.z)];
   if (!d.active) return;
   const int site = blockIdx.y;
   const int nrows = g.N1 * g.N2;
@@ -231,7 +238,7 @@ template <typename T, int N1, int LINES, bool INV>
 __global__ void __launch_bounds__(NT)
 fft_y(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> lds[LINES * N1];  // interleaved: point y of column l at y * LINES + l
-  const SoluteDesc& d = sd[blockIdx.z];
+  const SoluteDesc& d = sd[solute_of(g, INV ? 8 : 2, blockIdx.z, gridDim.z)];
   if (!d.active) return;
   const int site = blockIdx.y / g.N2;
   const int z = blockIdx.y - site * g.N2;
@@ -351,7 +358,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   extern __shared__ unsigned char smem[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
   // zorder 0: grid (solute, kx tile, ky); 1: grid (kx tile, ky, solute)
-  const int b = g.zorder ? blockIdx.z : blockIdx.x;
+  const int b = g.zorder ? solute_of(g, 4, blockIdx.z, gridDim.z) : solute_of(g, 4, blockIdx.x, gridDim.x);
   const SoluteDesc& d = sd[b];
   if (!d.active) return;
   const int m = g.m;
@@ -575,6 +582,7 @@ __device__ __forceinline__ double warp_max(double v) {
 }
 
 // --------------------------- 3b. zstar with the spectra held in registers
+// Mode 3: forward z FFTs only; mode 4: inverse z FFTs only.
 // The solute residual's k-space step (modes 0-7 of zstar, special 0) for a
 // compile-time site count MS: the z lines of one site at a time pass
 // through LDS (LINES * N2 points, interleaved), and each thread keeps its PPT
@@ -597,7 +605,8 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
   constexpr int PTS = N2 * LINES;
   constexpr int PPT = (PTS + NT - 1) / NT;
   __shared__ cpx<T> lds[PTS];  // interleaved: point z of line l at z * LINES + l
-  const int b = (NYQ || g.zorder) ? blockIdx.z : blockIdx.x;
+  const int b = (NYQ || g.zorder) ? solute_of(g, 4, blockIdx.z, gridDim.z)
+                                  : solute_of(g, 4, blockIdx.x, gridDim.x);
   const SoluteDesc& d = sd[b];
   if (!d.active) return;
   // NYQ: lines are the rows y0 + l of the Nyquist column kx = N0/2 (grid
@@ -628,6 +637,12 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
           if (nl == LINES || l < nl) v[p] = ldc(Ws + z * zs + l * ls);
         }
       }
+      if (d.mode == 4) {  // inverse only: the loaded spectrum goes to the inverse pass
+#pragma unroll
+        for (int p = 0; p < PPT; ++p)
+          if ((tid + p * NT) % LINES >= nl) v[p] = cpx<T>{T(0), T(0)};
+        continue;
+      }
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
         const int e = tid + p * NT;
@@ -653,7 +668,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
 #pragma unroll
   for (int p = 0; p < PPT; ++p) {
     const int e = tid + p * NT;
-    if (!(PPT * NT == PTS || e < PTS)) continue;
+    if (d.mode == 4 || !(PPT * NT == PTS || e < PTS)) continue;
     const int z = e / LINES, l = e - z * LINES;
     if (l >= nl) continue;
     const int64_t kk = base + z * zs + l * ls;  // k-grid index of this point
@@ -740,7 +755,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   constexpr int LSM = lstride_for(M);
   __shared__ cpx<T> zb[ROWS * LSM];
   __shared__ double red[NPART][NT / 64];
-  const SoluteDesc& d = sd[blockIdx.z];
+  const SoluteDesc& d = sd[solute_of(g, 16, blockIdx.z, gridDim.z)];
   if (!d.active) return;
   const int site = blockIdx.y;
   const int nrows = g.N1 * g.N2;
@@ -1106,6 +1121,45 @@ field_bv(const SitesDesc* __restrict__ sites, GridArgs g, const double* __restri
       bv64[blockIdx.y][i * g.P + e] = b;
       if (bv32) bv32[blockIdx.y][i * g.P + e] = (float)b;
     }
+  }
+}
+
+// Trigonometric interpolation of one solute's spectrum from the grid
+// N/2 to N in the same box: Wf(k) = Wc(k) scale at every frequency the
+// coarse grid holds, zero above.  The coarse Nyquist planes of the full y
+// and z axes go in halves to +N_c/2 and -N_c/2, that of the half x axis in
+// half to kx = N0_c/2 (the c2r mirrors it to -N0_c/2).  scale = 1 / P_c
+// with the unnormalised inverse of the fine pipeline.  Grid: x = chunks
+// of Kf, y = site.
+__device__ __forceinline__ int up_index(int kf, int nf, int nc, double& w) {
+  const int h = nc / 2;
+  w = 1.0;
+  if (kf < h) return kf;
+  if (kf == h || kf == nf - h) { w = 0.5; return h; }
+  if (kf > nf - h) return kf - nf + nc;
+  return -1;
+}
+__global__ void __launch_bounds__(NT)
+spec_upsample(const cpx<double>* __restrict__ wc, cpx<double>* __restrict__ wf, GridArgs g,
+              int N0c, int N1c, int N2c, int NHc, int64_t Kc, double scale) {
+  const int s = blockIdx.y;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.K; e += stride) {
+    const int kx = (int)(e % g.NH);
+    const int ky = (int)((e / g.NH) % g.N1);
+    const int kz = (int)(e / ((int64_t)g.NH * g.N1));
+    cpx<double> v = {0.0, 0.0};
+    double wx = 1.0, wy, wz;
+    const int hx = N0c / 2;
+    const int cx = kx < hx ? kx : (kx == hx ? (wx = 0.5, hx) : -1);
+    const int cy = up_index(ky, g.N1, N1c, wy);
+    const int cz = up_index(kz, g.N2, N2c, wz);
+    if (cx >= 0 && cy >= 0 && cz >= 0) {
+      const cpx<double> c = wc[s * Kc + ((int64_t)cz * N1c + cy) * NHc + cx];
+      const double w = wx * wy * wz * scale;
+      v = cpx<double>{c.re * w, c.im * w};
+    }
+    wf[s * g.K + e] = v;
   }
 }
 

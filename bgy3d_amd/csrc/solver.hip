@@ -1663,6 +1663,7 @@ int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
   g.rho = pd->rho;
   g.renorm = 0;
   g.zorder = env_int("RISM3D_ZSTAR_ORDER", 1) ? 1 : 0;
+  g.rev = env_int("RISM3D_REV", 0) & 31;
   g.closure = pd->closure;
   g.pse_order = pd->closure == RISM3D_CLOSURE_KH ? 1 : (pd->closure >= RISM3D_CLOSURE_PSE0 ? pd->closure - RISM3D_CLOSURE_PSE0 : 0);
   for (int i = 0; i < m; ++i) g.q[i] = solvent[i].charge;
@@ -1800,8 +1801,11 @@ int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
   return res->status;
 }
 
-int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
-        std::vector<double>* t_final) {
+// spec_out (a nested coarse solve): the forward spectra of the converged
+// t, per solute (empty where it did not converge), moved out of the
+// problem's buffers.
+int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
+             std::vector<double>* t_final, std::vector<DevBuf<cpx<double>>>* spec_out) {
   int v = validate(in);
   if (v != RISM3D_OK) {
     for (int b = 0; b < in.B; ++b) res[b].status = v;
@@ -1812,7 +1816,6 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     for (int b = 0; b < in.B; ++b) res[b].status = RISM3D_NO_DEVICE;
     return RISM3D_NO_DEVICE;
   }
-  std::lock_guard<std::mutex> lock(g_mutex);
   HIPCHK(hipSetDevice(in.opt.device));
   const rism3d_problem& pd = *in.pd;
   Problem pb;
@@ -1852,6 +1855,7 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   g.beta = pd.beta;
   g.renorm = (in.opt.renorm && !in.opt.chi3d) ? 1 : 0;
   g.zorder = env_int("RISM3D_ZSTAR_ORDER", 1) ? 1 : 0;
+  g.rev = env_int("RISM3D_REV", 0) & 31;
   g.closure = pd.closure;
   g.pse_order = pd.closure == RISM3D_CLOSURE_KH ? 1 : (pd.closure >= RISM3D_CLOSURE_PSE0 ? pd.closure - RISM3D_CLOSURE_PSE0 : 0);
   for (int i = 0; i < in.m; ++i) g.q[i] = in.solvent[i].charge;
@@ -2002,6 +2006,66 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     HIPCHK(hipGetLastError());
   }
 
+  // coarse start: the same problem on N/2 first, its t interpolated to N
+  // (one level: starting the N/2 solve from N/4 as well measured no faster)
+  bool coarse = in.opt.coarse_tol > 0 && !in.opt.chi3d && use_zreg(g) && !t_init &&
+                !spec_out && !in.j_dt && !in.t_eval;
+  for (int d = 0; d < 3; ++d)
+    coarse = coarse && pd.N[d] % 2 == 0 && supported_n(pd.N[d] / 2);
+  if (coarse) {
+    rism3d_problem pdc = pd;
+    for (int d = 0; d < 3; ++d) pdc.N[d] = pd.N[d] / 2;
+    pdc.norm_tol = std::max(pd.norm_tol, in.opt.coarse_tol);
+    Inputs ic = in;
+    ic.pd = &pdc;
+    ic.opt.coarse_tol = 0;
+    ic.opt.polish_iters = 0;
+    ic.opt.mixed_tol = std::max(in.opt.mixed_tol, in.opt.coarse_tol);
+    std::vector<rism3d_result> rc(B);
+    std::memset(rc.data(), 0, B * sizeof(rism3d_result));
+    std::vector<DevBuf<cpx<double>>> spec;
+    const bool prof = g_prof.on;  // kernel timing covers the grid N only
+    g_prof.on = false;
+    run_impl(ic, rc.data(), nullptr, nullptr, &spec);
+    g_prof.on = prof;
+    const int N0c = pdc.N[0], N1c = pdc.N[1], N2c = pdc.N[2];
+    const int NHc = (N0c / 2 + 1 + 15) / 16 * 16;
+    const int64_t Kc = (int64_t)NHc * N1c * N2c;
+    const double Pc = (double)N0c * N1c * N2c;
+    std::vector<SoluteDesc> sd(B);
+    std::memset(sd.data(), 0, B * sizeof(SoluteDesc));
+    const int ck = (int)std::min<int64_t>(4096, (g.K + NT - 1) / NT);
+    bool any = false;
+    for (int b = 0; b < B; ++b) {
+      if (b >= (int)spec.size() || !spec[b].p) continue;
+      auto& s = pb.st[b];
+      hipLaunchKernelGGL(spec_upsample, dim3(ck, m), dim3(NT), 0, pb.s, spec[b].p, s.W.p, g,
+                         N0c, N1c, N2c, NHc, Kc, 1.0 / Pc);
+      // inverse pass (zstar mode 4, y, c2r with r = z) into t
+      sd[b].t = s.t64.p;
+      sd[b].r = s.t64.p;
+      sd[b].bv = s.bv64.p;
+      sd[b].W = s.W.p;
+      sd[b].Fs = s.fs64.p;
+      sd[b].part = s.part.p;
+      sd[b].mode = 4;
+      sd[b].active = 1;
+      any = true;
+    }
+    if (any) {
+      upload_sd(pb, sd);
+      using T = double;
+      constexpr int LN = y_lines<T>();
+      const dim3 gx(resid_grid_x(g), m, B);
+      const dim3 gy((g.NX + LN - 1) / LN, m * g.N2, B);
+      launch_zstar<T>(g.N2, g.NX, g.N1, B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p);
+      launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
+      launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, 0);
+      HIPCHK(hipStreamSynchronize(pb.s));
+      HIPCHK(hipGetLastError());
+    }
+  }
+
   if (in.j_dt) {
     // one Jacobian application at t (jacobian_t1, hnc3d.c:1290-1362)
     auto& s0 = pb.st[0];
@@ -2139,6 +2203,36 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
       }
     }
   }
+  if (spec_out) {
+    // forward spectrum of each converged t (closure pass with c = t, y,
+    // zstar mode 3), handed to the caller with its buffer
+    std::vector<SoluteDesc> sd(B);
+    std::memset(sd.data(), 0, B * sizeof(SoluteDesc));
+    for (int b = 0; b < B; ++b) {
+      auto& s = pb.st[b];
+      sd[b].t = s.t64.p;
+      sd[b].bv = s.bv64.p;
+      sd[b].W = s.W.p;
+      sd[b].Fs = s.fs64.p;
+      sd[b].part = s.part.p;
+      sd[b].mode = 3;
+      sd[b].active = s.status == RISM3D_OK ? 1 : 0;
+    }
+    upload_sd(pb, sd);
+    using T = double;
+    constexpr int LN = y_lines<T>();
+    const dim3 gx(resid_grid_x(g), m, B);
+    const dim3 gy((g.NX + LN - 1) / LN, m * g.N2, B);
+    launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, 0);
+    launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p);
+    launch_zstar<T>(g.N2, g.NX, g.N1, B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p);
+    HIPCHK(hipStreamSynchronize(pb.s));
+    HIPCHK(hipGetLastError());
+    spec_out->resize(B);
+    for (int b = 0; b < B; ++b)
+      if (sd[b].active) (*spec_out)[b] = std::move(pb.st[b].W);
+    return RISM3D_OK;
+  }
   for (int b = 0; b < B; ++b)
     if (res[b].info && pb.st[b].status == RISM3D_OK) info_observables(pb, in, b, sites[b], res[b].info);
   gradients(pb, in, res, cf, sites);
@@ -2146,6 +2240,12 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   int worst = RISM3D_OK;
   for (int b = 0; b < B; ++b) worst = std::max(worst, res[b].status);
   return worst;
+}
+
+int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
+        std::vector<double>* t_final) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  return run_impl(in, res, t_init, t_final, nullptr);
 }
 
 }  // namespace

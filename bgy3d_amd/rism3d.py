@@ -66,7 +66,8 @@ class Options(ctypes.Structure):
                 ("device", ctypes.c_int), ("dielectric", ctypes.c_double),
                 ("continuation", ctypes.c_int),
                 ("chi3d", ctypes.POINTER(ctypes.c_double)),
-                ("site_rho", ctypes.POINTER(ctypes.c_double))]
+                ("site_rho", ctypes.POINTER(ctypes.c_double)),
+                ("coarse_tol", ctypes.c_double)]
 
 
 class Result(ctypes.Structure):
@@ -226,7 +227,7 @@ def make_options(settings=None, **kw):
                        ("mixed-tol", "mixed_tol"), ("diis-depth", "diis_depth"),
                        ("diis-step", "diis_step"), ("verbose", "verbose"),
                        ("device", "device"), ("dielectric", "dielectric"),
-                       ("continuation", "continuation")):
+                       ("continuation", "continuation"), ("coarse-tol", "coarse_tol")):
         if key in s:
             v = s[key]
             if key == "precision" and isinstance(v, str):

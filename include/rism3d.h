@@ -148,6 +148,16 @@ typedef struct rism3d_options {
                           h_i = sum_j c_j chi_ji, tau_i = sum_j q_j chi_ji,
                           and mu / the PMV factor weight site i by rho_i.
                           Requires chi_k; no gradients (UNSUPPORTED)       */
+  double coarse_tol;   /* 0: start from t = 0 (hnc3d.c:1845).  > 0 (no
+                          counterpart in the reference): first solve the
+                          same problem on the grid N/2 in the same box to
+                          max|F| <= coarse_tol, then start from that t,
+                          trigonometrically interpolated to N.  Same root,
+                          fewer residuals on N; iterations counts those on
+                          N only.  Applies when every N[d] / 2 is a
+                          supported size, the solvent has 3 or 5 sites and
+                          neither chi3d nor a restart token is given;
+                          otherwise ignored.                               */
 } rism3d_options;
 
 typedef struct rism3d_result {
