@@ -130,13 +130,6 @@ __device__ __forceinline__ void stbf4(uint16_t* p, BF4 q) {
   *(G*)p = x;
 }
 
-// Solute of a residual-kernel workgroup: grid index idx of n, reversed when
-// bit `bit` of g.rev is set, so that a kernel can start on the solutes the
-// previous kernel of the chain wrote last.
-__device__ __forceinline__ int solute_of(const GridArgs& g, int bit, int idx, int n) {
-  return (g.rev & bit) ? n - 1 - idx : idx;
-}
-
 // ------------------------------------------- 1. closure fused with x r2c
 // t = sum_j coef_j u_j (the MDIIS step; written back as the iterate) or the
 // given t, then c = closure(beta v, t), then the x-axis r2c of c.
@@ -149,7 +142,7 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
   constexpr int ROWS = rows_for(M);
   constexpr int LSM = lstride_for(M);
   __shared__ cpx<T> lds[ROWS * LSM];
-  const SoluteDesc& d = sd[solute_of(g, 1, blockIdx.z, gridDim.z)];
+  const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
   const int site = blockIdx.y;
   const int nrows = g.N1 * g.N2;
@@ -238,7 +231,7 @@ template <typename T, int N1, int LINES, bool INV>
 __global__ void __launch_bounds__(NT)
 fft_y(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> lds[LINES * N1];  // interleaved: point y of column l at y * LINES + l
-  const SoluteDesc& d = sd[solute_of(g, INV ? 8 : 2, blockIdx.z, gridDim.z)];
+  const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
   const int site = blockIdx.y / g.N2;
   const int z = blockIdx.y - site * g.N2;
@@ -358,7 +351,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   extern __shared__ unsigned char smem[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
   // zorder 0: grid (solute, kx tile, ky); 1: grid (kx tile, ky, solute)
-  const int b = g.zorder ? solute_of(g, 4, blockIdx.z, gridDim.z) : solute_of(g, 4, blockIdx.x, gridDim.x);
+  const int b = g.zorder ? blockIdx.z : blockIdx.x;
   const SoluteDesc& d = sd[b];
   if (!d.active) return;
   const int m = g.m;
@@ -605,8 +598,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
   constexpr int PTS = N2 * LINES;
   constexpr int PPT = (PTS + NT - 1) / NT;
   __shared__ cpx<T> lds[PTS];  // interleaved: point z of line l at z * LINES + l
-  const int b = (NYQ || g.zorder) ? solute_of(g, 4, blockIdx.z, gridDim.z)
-                                  : solute_of(g, 4, blockIdx.x, gridDim.x);
+  const int b = (NYQ || g.zorder) ? blockIdx.z : blockIdx.x;
   const SoluteDesc& d = sd[b];
   if (!d.active) return;
   // NYQ: lines are the rows y0 + l of the Nyquist column kx = N0/2 (grid
@@ -755,7 +747,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   constexpr int LSM = lstride_for(M);
   __shared__ cpx<T> zb[ROWS * LSM];
   __shared__ double red[NPART][NT / 64];
-  const SoluteDesc& d = sd[solute_of(g, 16, blockIdx.z, gridDim.z)];
+  const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
   const int site = blockIdx.y;
   const int nrows = g.N1 * g.N2;

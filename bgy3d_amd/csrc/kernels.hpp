@@ -69,7 +69,6 @@ struct GridArgs {
   int closure;         // enum rism3d_closure
   int pse_order;       // n of PSE-n, 1 for KH
   int zorder;          // zstar grid order (see zstar)
-  int rev;             // bit k: residual kernel k walks the solutes last to first
   double q[MAX_SITES]; // solvent charges
   // zstar variants (0 solute residual): 1 solvent-solvent OZ on site-site
   // pairs (msol sites, pair p = i(i+1)/2 + j), 2 Poisson potential of a

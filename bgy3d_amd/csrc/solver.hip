@@ -1663,7 +1663,6 @@ int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
   g.rho = pd->rho;
   g.renorm = 0;
   g.zorder = env_int("RISM3D_ZSTAR_ORDER", 1) ? 1 : 0;
-  g.rev = env_int("RISM3D_REV", 0) & 31;
   g.closure = pd->closure;
   g.pse_order = pd->closure == RISM3D_CLOSURE_KH ? 1 : (pd->closure >= RISM3D_CLOSURE_PSE0 ? pd->closure - RISM3D_CLOSURE_PSE0 : 0);
   for (int i = 0; i < m; ++i) g.q[i] = solvent[i].charge;
@@ -1855,7 +1854,6 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   g.beta = pd.beta;
   g.renorm = (in.opt.renorm && !in.opt.chi3d) ? 1 : 0;
   g.zorder = env_int("RISM3D_ZSTAR_ORDER", 1) ? 1 : 0;
-  g.rev = env_int("RISM3D_REV", 0) & 31;
   g.closure = pd.closure;
   g.pse_order = pd.closure == RISM3D_CLOSURE_KH ? 1 : (pd.closure >= RISM3D_CLOSURE_PSE0 ? pd.closure - RISM3D_CLOSURE_PSE0 : 0);
   for (int i = 0; i < in.m; ++i) g.q[i] = in.solvent[i].charge;
