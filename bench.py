@@ -146,7 +146,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nsolutes", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--diis-depth", type=int, default=8)
+    ap.add_argument("--diis-depth", type=int, default=10)
     ap.add_argument("--diis-step", type=float, default=0.5)
     ap.add_argument("--polish", type=int, default=3)
     ap.add_argument("--skip-natural", action="store_true",
