@@ -80,3 +80,26 @@ def test_coarse_start_ignored_where_half_grid_unsupported(chi160):
     b = R.hnc3d_run_solute(sol, W, dict(s, **{"coarse-tol": 1e-3}), chi=chi160)
     assert a["free-energy"] == b["free-energy"] and a["iterations"] == b["iterations"]
     assert np.array_equal(a["GUV"], b["GUV"])
+
+
+def test_history_widths_reach_one_root(chi160):
+    """MDIIS depths 8, 10 and 12 run the closure and residual kernels with
+    8, 10 and 16 history slots unrolled; FP64 solves reach one root, and the
+    mixed solve at depth 10 stays within the mixed bar."""
+    print("case history widths 8 / 10 / 16 at 64^3", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd import synthetic as S
+    sol = S.solute(11, n=9)
+    s = dict(BASE, N=64, L=12.0, closure="KH")
+    runs = {d: R.hnc3d_run_solute(sol, W, dict(s, **{"diis-depth": d}), chi=chi160)
+            for d in (8, 10, 12)}
+    mixed = R.hnc3d_run_solute(sol, W, dict(s, precision="mixed", **{"diis-depth": 10}),
+                               chi=chi160)
+    for d, r in runs.items():
+        print(f"  depth {d}: mu {r['free-energy']:.10f} residuals {r['iterations']}", flush=True)
+        assert r["status"] == "OK"
+        assert abs(r["free-energy"] - runs[8]["free-energy"]) < 1e-7
+        assert float(np.max(np.abs(r["GUV"] - runs[8]["GUV"]))) < 1e-7
+    print(f"  mixed depth 10: mu {mixed['free-energy']:.8f}", flush=True)
+    assert mixed["status"] == "OK"
+    assert abs(mixed["free-energy"] - runs[8]["free-energy"]) < 0.01
