@@ -49,6 +49,9 @@ def parse(argv):
                     default="mdiis")
     ap.add_argument("--lambda", dest="lam", type=float, default=0.02)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--coarse-tol", type=float, default=0.0,
+                    help="solve on the half grid to this residual first and start from "
+                         "its solution (no counterpart in the reference; 0: off)")
     ap.add_argument("--chi-upscale", action="store_true")
     ap.add_argument("--solvent-3d", action="store_true")
     ap.add_argument("--verbose", action="store_true")
@@ -64,7 +67,7 @@ def main(argv=None):
     settings = {"L": a.L, "N": a.N, "rho": a.rho, "beta": a.beta, "norm-tol": a.norm_tol,
                 "max-iter": a.max_iter, "closure": a.closure, "dielectric": a.dielectric,
                 "lambda": a.lam, "snes-solver": a.snes_solver, "precision": a.precision,
-                "verbose": int(a.verbose)}
+                "verbose": int(a.verbose), "coarse-tol": a.coarse_tol}
     if a.rmax is not None:
         settings["rmax"] = a.rmax
     if a.nrad is not None:

@@ -238,6 +238,15 @@ def test_cli_energy_hnc(capsys):
     print(f"  printed {out}", flush=True)
     assert rc == 0
     assert abs(float(out) - (-6.349052)) < 1e-6
+    # the same README number from the coarse start (96^3 from 48^3)
+    rc = cli.main(["energy", "--hnc", "--norm-tol", "1e-12", "--dielectric", "78.4",
+                   "--rho", "0.0333295", "--beta", "1.6889", "--L", "10", "--N", "96",
+                   "--solvent", "water, PR-SPC/E", "--solute", "water, PR-SPC/E",
+                   "--chi-upscale", "--coarse-tol", "1e-3"])
+    out = capsys.readouterr().out.strip().splitlines()[-1]
+    print(f"  printed {out} (coarse start)", flush=True)
+    assert rc == 0
+    assert abs(float(out) - (-6.349052)) < 1e-6
 
 
 def test_forces_scm_anchors_gpu():
