@@ -370,6 +370,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
   }
   __syncthreads();
   const int nl = min(LINES, g.NX - kx0);  // valid kx columns of this tile
+  if (d.mode != 4) {  // mode 4: inverse z FFTs of the loaded spectrum only
   for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, false>(lds + s * LINES * LS, LS, nl, tw);
   if (d.mode == 3) {  // forward spectrum only
     for (int s = 0; s < m; ++s) {
@@ -552,6 +553,7 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
     }
   }
   }  // g.special
+  }  // d.mode != 4
   __syncthreads();
   for (int s = 0; s < m; ++s) fft_lines<T, N2, LINES, NT, true>(lds + s * LINES * LS, LS, nl, tw);
   for (int s = 0; s < m; ++s) {

@@ -2023,7 +2023,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
 
   // coarse start: the same problem on N/2 first, its t interpolated to N
   // (one level: starting the N/2 solve from N/4 as well measured no faster)
-  bool coarse = in.opt.coarse_tol > 0 && !in.opt.chi3d && use_zreg(g) && !t_init &&
+  bool coarse = in.opt.coarse_tol > 0 && !in.opt.chi3d && g.special == 0 && !t_init &&
                 !spec_out && !in.j_dt && !in.t_eval;
   for (int d = 0; d < 3; ++d)
     coarse = coarse && pd.N[d] % 2 == 0 && supported_n(pd.N[d] / 2);

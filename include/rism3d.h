@@ -155,9 +155,8 @@ typedef struct rism3d_options {
                           trigonometrically interpolated to N.  Same root,
                           fewer residuals on N; iterations counts those on
                           N only.  Applies when every N[d] / 2 is a
-                          supported size, the solvent has 3 or 5 sites and
-                          neither chi3d nor a restart token is given;
-                          otherwise ignored.                               */
+                          supported size and neither chi3d nor a restart
+                          token is given; otherwise ignored.               */
 } rism3d_options;
 
 typedef struct rism3d_result {

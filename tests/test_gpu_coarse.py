@@ -103,3 +103,24 @@ def test_history_widths_reach_one_root(chi160):
     print(f"  mixed depth 10: mu {mixed['free-energy']:.8f}", flush=True)
     assert mixed["status"] == "OK"
     assert abs(mixed["free-energy"] - runs[8]["free-energy"]) < 0.01
+
+
+def test_coarse_start_one_site_solvent_forces_anchors():
+    """A 1-site solvent runs the all-sites-in-LDS zstar (inverse-only mode
+    for the interpolated start): test/forces.scm:28-30 from the coarse start
+    (96^3 from 48^3, HNC, OW solvent, chi from the 1D producer) to the same
+    1e-8 as the cold start.  These uncharged solutes converge in about 25
+    residuals from t = 0, so the start saves little (25 -> 22 for OW, 27 ->
+    29 for OW2-UA): only the root is checked."""
+    print("case coarse start forces-scm-96 OW solvent", flush=True)
+    from bgy3d_amd import rism3d as R
+    s = {"N": 96, "L": 10.0, "rho": 0.0333295, "beta": 1.6889, "closure": "HNC",
+         "norm-tol": 1e-12, "max-iter": 2000}
+    for name, ref in (("OW", 6.86940765594207), ("OW2-UA", 5.82278007673305),
+                      ("OW2", 14.0074968008389)):
+        cold = R.hnc3d_run_solute(name, "OW", s, chi=None)
+        warm = R.hnc3d_run_solute(name, "OW", dict(s, **{"coarse-tol": 1e-3}), chi=None)
+        print(f"  {name}: mu {warm['free-energy']:.12f} reference {ref} residuals "
+              f"{cold['iterations']} -> {warm['iterations']}", flush=True)
+        assert warm["status"] == "OK"
+        assert abs(warm["free-energy"] - ref) < 1e-8
