@@ -2030,7 +2030,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   if (coarse) {
     rism3d_problem pdc = pd;
     for (int d = 0; d < 3; ++d) pdc.N[d] = pd.N[d] / 2;
-    pdc.norm_tol = std::max(pd.norm_tol, in.opt.coarse_tol);
+    pdc.norm_tol = in.opt.coarse_tol;
     Inputs ic = in;
     ic.pd = &pdc;
     ic.opt.coarse_tol = 0;

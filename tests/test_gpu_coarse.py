@@ -124,3 +124,29 @@ def test_coarse_start_one_site_solvent_forces_anchors():
               f"{cold['iterations']} -> {warm['iterations']}", flush=True)
         assert warm["status"] == "OK"
         assert abs(warm["free-energy"] - ref) < 1e-8
+
+
+def test_interpolated_start_matches_checker(chi160):
+    """The start the coarse stage hands to N, point by point: a solve with
+    norm-tol 1e9 stops at its first residual and returns that start, which
+    must be the trigonometric interpolation (tests/trig_upsample.py, pinned
+    on band-limited fields) of the standalone N/2 solve to the same
+    coarse_tol."""
+    print("case coarse start interpolated t vs checker 32^3 -> 64^3", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd import synthetic as S
+    from trig_upsample import upsample
+    sol = S.solute(3, n=12)
+    s = dict(BASE, N=64, L=12.0, closure="KH")
+    coarse = R.hnc3d_run_solute(sol, W, dict(s, N=32, **{"norm-tol": 1e-3}), chi=chi160,
+                                want_t=True)
+    start = R.hnc3d_run_solute(sol, W, dict(s, **{"norm-tol": 1e9, "coarse-tol": 1e-3}),
+                               chi=chi160, want_t=True)
+    want = upsample(coarse["t"])
+    err = float(np.max(np.abs(start["t"] - want)))
+    scale = float(np.max(np.abs(want)))
+    print(f"  coarse residuals {coarse['iterations']}, start returned after "
+          f"{start['iterations']}; max|t - interp| {err:.2e} of {scale:.2e}", flush=True)
+    assert coarse["status"] == "OK" and start["status"] == "OK"
+    assert start["iterations"] <= 1
+    assert err <= 1e-12 * scale * 100
