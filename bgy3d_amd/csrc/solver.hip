@@ -2039,10 +2039,14 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     std::vector<rism3d_result> rc(B);
     std::memset(rc.data(), 0, B * sizeof(rism3d_result));
     std::vector<DevBuf<cpx<double>>> spec;
-    const bool prof = g_prof.on;  // kernel timing covers the grid N only
-    g_prof.on = false;
-    run_impl(ic, rc.data(), nullptr, nullptr, &spec);
-    g_prof.on = prof;
+    {
+      struct ProfPause {  // kernel timing covers the grid N only
+        bool saved = g_prof.on;
+        ProfPause() { g_prof.on = false; }
+        ~ProfPause() { g_prof.on = saved; }
+      } pause;
+      run_impl(ic, rc.data(), nullptr, nullptr, &spec);
+    }
     const int N0c = pdc.N[0], N1c = pdc.N[1], N2c = pdc.N[2];
     const int NHc = (N0c / 2 + 1 + 15) / 16 * 16;
     const int64_t Kc = (int64_t)NHc * N1c * N2c;
