@@ -28,9 +28,11 @@ Pinning: README:20-31 (-6.349052 renorm, -6.297170 --no-renorm for
 PR-SPC/E water at --L 10 --N 96) reproduced to 1e-7 with chi from 1D DRISM
 on the grid hnc3d_solute_solve builds itself when chi_fft_buf is NULL
 (upscale(): rmax = 40, nrad = 1536, hnc3d.c:772-783);
-test/out/run-ions-out:56-110 (charged single sites, KH/PSE-n/HNC) to the
-reference's own reproducibility floor; grid invariants (translation by whole
-cells); in tests/test_oracle_*.py.
+test/out/run-ions-out:56-110 (nine charged single sites, KH/PSE-n/HNC):
+per closure every ion to the printed 4 decimals after one constant of the
+reference's own solvent chi run, which the 1D rows show as well (13-15
+times smaller); a dropped tau(0) F(0) term fails that by 0.14 kcal/mol;
+grid invariants; in tests/test_oracle_*.py.
 """
 import math
 import os

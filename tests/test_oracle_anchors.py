@@ -5,10 +5,8 @@ README:9-16   1D DRISM + 1D solute RISM, PR-SPC/E water in itself,
 README:20-31  3D-RISM, same molecules, --L 10 --N 96: mu = -6.349052
               (renorm), -6.297170 (--no-renorm).  chi from 1D DRISM on the
               grid hnc3d_solute_solve builds itself (rmax 40, nrad 1536).
-test/out/run-ions-out  1D rows for Na+/Cl- KH in cSPC/E water, agreement
-              to the reference's own reproducibility floor (the golden
-              file's KH and PSE1 rows differ by 2e-4 although the closures
-              are identical).
+(The net-charged rows of test/out/run-ions-out are pinned in
+test_oracle_run_ions.py.)
 Also: a cyclic permutation of the axes (an exact symmetry of a cubic grid
 with the same N and L on every axis) leaves mu unchanged for an uncharged
 solute; a transposed index anywhere in the grid code would break it.  (With
@@ -18,7 +16,7 @@ there -- reference behaviour, reproduced by the GPU path.)
 """
 import numpy as np
 
-from conftest import CHI_CSPCE_KH, CHI_W, load_chi
+from conftest import CHI_W, load_chi
 from oracle import closures as C
 from oracle import rism1d, rism3d
 from bgy3d_amd.solutes import find_molecule
@@ -49,14 +47,6 @@ def test_readme_3d_norenorm():
     pd = rism3d.Problem.from_cli(10.0, 96, 1.6889, 0.0333295, rmax=40.0, nrad=1536)
     out = rism3d.solute_solve(pd, W, W, chi, renorm=False, atol=1e-10)
     assert abs(out["mu"] - (-6.297170)) < 1e-6
-
-
-def test_run_ions_1d_kh():
-    chi = load_chi(CHI_CSPCE_KH)
-    sv = find_molecule("water, cSPC/E")
-    for ion, ref in (("Na+, JC08, SPC/E", -84.9765), ("Cl-, JC08, SPC/E", -78.5084)):
-        uv = rism1d.rism_uv(C.KH, 4096, 160.0, 1.6889, 0.0333295, sv, chi, find_molecule(ion))
-        assert abs(uv["mu"] - ref) < 1e-3, (ion, uv["mu"])
 
 
 def test_axis_permutation_symmetry():
