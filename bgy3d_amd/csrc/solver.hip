@@ -22,6 +22,13 @@
 
 #include "kernels.hip"
 
+// The fixed device scratch of rism3d_residual_noalloc: every device buffer
+// of that entry point is carved from this array, of which each device holds
+// its own copy from module load to process end.  Host code reaches it only
+// through the address hipGetSymbolAddress returns on each device; the name
+// itself is not a device address on the host.
+static __device__ __align__(256) unsigned char scratch[(size_t)RISM3D_NOALLOC_SCRATCH_MB << 20];
+
 namespace rism {
 namespace {
 
@@ -226,7 +233,25 @@ void cache_empty(int dev) {  // dev < 0: every device
   }
   (void)hipSetDevice(cur);
 }
+// A bump arena over memory the library already holds (the fixed scratch).
+// While a thread has one installed, dev_alloc carves its buffers from it and
+// dev_free leaves them there: the arena is released as a whole when the
+// call that installed it returns.  A request beyond its end throws
+// ScratchFull before anything is written.
+struct ScratchFull {};
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+};
+thread_local Arena* t_arena = nullptr;
+
 void* dev_alloc(size_t bytes) {
+  if (Arena* a = t_arena) {
+    const size_t off = (a->used + 255) & ~(size_t)255;
+    if (off > a->size || bytes > a->size - off) throw ScratchFull{};
+    a->used = off + bytes;
+    return a->base + off;
+  }
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   MemCache& c = mem_cache();
@@ -249,6 +274,8 @@ void* dev_alloc(size_t bytes) {
   return p;
 }
 void dev_free(void* p, size_t bytes) {
+  if (const Arena* a = t_arena)
+    if ((char*)p >= a->base && (char*)p < a->base + a->size) return;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return;
   MemCache& c = mem_cache();
@@ -487,11 +514,17 @@ struct Problem {
     }
   } graphs;
   int use_graphs = env_int("RISM3D_GRAPHS", 1);
+  int noalloc = 0;  // buffers from the fixed scratch: no pinned staging, no graphs
 };
 
 // Descriptors to the device through pinned staging (stream-ordered).
 void upload_sd(Problem& pb, const std::vector<SoluteDesc>& sd) {
   const size_t bytes = sd.size() * sizeof(SoluteDesc);
+  if (pb.noalloc) {  // from pageable memory, complete before sd can change
+    HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), bytes, hipMemcpyHostToDevice, pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+    return;
+  }
   int k = 0;
   void* h = pb.up.slot(bytes, &k);
   std::memcpy(h, sd.data(), bytes);
@@ -1295,6 +1328,8 @@ struct Inputs {
   double* r_out = nullptr;         // residual_only: F(t) out
   const double* j_dt = nullptr;    // jacobian: dt in (t in t_eval), J dt out in r_out
   int j_flag = 0;
+  int noalloc = 0;                 // every device buffer from the installed arena
+  hipStream_t stream = nullptr;    // a stream the caller keeps (else one per call)
 };
 
 int validate(const Inputs& in) {
@@ -1836,8 +1871,11 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   const rism3d_problem& pd = *in.pd;
   Problem pb;
   pb.B = in.B;
-  pb.D = in.opt.solver == RISM3D_SOLVER_MDIIS ? in.opt.diis_depth : 1;
+  // one residual or Jacobian application uses history slot 0 only
+  pb.D = (in.opt.solver == RISM3D_SOLVER_MDIIS && !in.t_eval) ? in.opt.diis_depth : 1;
   pb.mixed = in.opt.precision != RISM3D_FP64;
+  pb.noalloc = in.noalloc;
+  if (in.noalloc) pb.use_graphs = 0;
   // BF16 residual history with defect correction only: the FP32-only study
   // iterates t itself to its floor, where the Gram rounding would show
   if (in.opt.precision != RISM3D_MIXED) pb.r16 = 0;
@@ -1876,11 +1914,18 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   for (int i = 0; i < in.m; ++i) g.q[i] = in.solvent[i].charge;
   pb.mP = (int64_t)in.m * g.P;
 
-  HIPCHK(hipStreamCreateWithFlags(&pb.s, hipStreamNonBlocking));
+  if (in.stream)
+    pb.s = in.stream;
+  else
+    HIPCHK(hipStreamCreateWithFlags(&pb.s, hipStreamNonBlocking));
   struct StreamGuard {  // idle before its buffers return to the cache
     hipStream_t s;
-    ~StreamGuard() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
-  } sg{pb.s};
+    bool own;
+    ~StreamGuard() {
+      (void)hipStreamSynchronize(s);
+      if (own) (void)hipStreamDestroy(s);
+    }
+  } sg{pb.s, in.stream == nullptr};
   make_twiddles(pb.tw, pb.N, pb.s);
 
   // Solvent kernel: chi - delta and the renormalisation tables
@@ -2267,6 +2312,39 @@ int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   return run_impl(in, res, t_init, t_final, nullptr);
 }
 
+// Device bytes one residual evaluation takes from the arena (the buffers
+// run_impl creates on the t_eval path with one history slot), each rounded
+// up to the arena's 256-byte alignment; the residual-workgroup partials
+// are bounded by one workgroup per (y, z) row and site.
+size_t noalloc_bytes(const rism3d_problem& pd, int m, int n, bool mixed) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t N0 = pd.N[0], N1 = pd.N[1], N2 = pd.N[2];
+  const size_t NH = (N0 / 2 + 1 + 15) / 16 * 16;
+  const size_t P = N0 * N1 * N2, K = NH * N1 * N2, mP = (size_t)m * P;
+  const size_t np = (size_t)m * (m + 1) / 2, nrad = (size_t)std::max(pd.nrad, 0);
+  size_t b = 0;
+  for (size_t len : {N0, N0 / 2, N1, N2}) b += al(len * 16) + al(len * 8);  // twiddles
+  b += al(np * nrad * 8) + al(m * nrad * 8);                       // 1D tables
+  b += al(np * K * 8) + al(np * K * 4) + al(m * K * 8) + al(m * K * 4);  // 3D tables
+  b += 4 * al(mP * 8) + al(m * K * 16) + al(K * 16);  // U, R, t, beta v; W; F s
+  if (mixed) b += 4 * al(mP * 4) + al(mP * 8) + al(K * 8);
+  b += al(N1 * N2 * m * NPART * 8) + al((size_t)std::max(n, 0) * 6 * 8);
+  b += al(sizeof(SoluteDesc)) + al(sizeof(SitesDesc)) + al(NPART * 8) + 3 * al(8);
+  b += al(3 * (size_t)m * 8) + 4 * al(8);  // setup descriptors
+  return b;
+}
+
+// Per device: the address of the fixed scratch and the stream of the
+// allocation-free entry point, set up on its first call on that device.
+struct NoallocDevice {
+  std::mutex mu;
+  std::map<int, std::pair<void*, hipStream_t>> dev;
+};
+NoallocDevice& noalloc_device() {
+  static NoallocDevice d;
+  return d;
+}
+
 }  // namespace
 }  // namespace rism
 
@@ -2405,6 +2483,77 @@ int rism3d_residual(const rism3d_problem* pd, int m, const rism3d_site* solvent,
   std::memset(&res, 0, sizeof res);
   try {
     return rism::run(in, &res, nullptr, nullptr);
+  } catch (const rism::HipFailure& e) {
+    return (e.err == hipErrorOutOfMemory) ? RISM3D_OOM : RISM3D_NO_DEVICE;
+  }
+}
+
+int rism3d_residual_noalloc(const rism3d_problem* pd, int m, const rism3d_site* solvent, int n,
+                            const rism3d_site* solute, const double* chi_k,
+                            const rism3d_options* opt, const double* t, double* r) {
+  if (!pd || !chi_k || !t || !r || !solute || !solvent || m < 1 || n < 1) return RISM3D_BAD_ARG;
+  rism::Inputs in;
+  in.pd = pd;
+  in.m = m;
+  in.solvent = solvent;
+  in.B = 1;
+  const rism3d_site* sol[1] = {solute};
+  int ns[1] = {n};
+  in.n_sites = ns;
+  in.solutes = sol;
+  in.chi_k = chi_k;
+  if (opt) in.opt = *opt; else rism3d_options_default(&in.opt);
+  in.opt.coarse_tol = 0;
+  in.t_eval = t;
+  in.r_out = r;
+  const size_t cap = sizeof(scratch);
+  const size_t need = rism::noalloc_bytes(*pd, m, n, in.opt.precision != RISM3D_FP64);
+  if (need > cap) {
+    std::fprintf(stderr,
+                 "rism3d_residual_noalloc: this problem needs %zu bytes of device scratch, "
+                 "the fixed scratch holds %zu (RISM3D_NOALLOC_SCRATCH_MB)\n", need, cap);
+    return RISM3D_NO_SCRATCH;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || in.opt.device < 0 || ndev <= in.opt.device)
+    return RISM3D_NO_DEVICE;
+  rism3d_result res;
+  std::memset(&res, 0, sizeof res);
+  struct ArenaGuard {
+    ~ArenaGuard() { rism::t_arena = nullptr; }
+  } ag;
+  try {
+    void* base = nullptr;
+    {
+      rism::NoallocDevice& nd = rism::noalloc_device();
+      std::lock_guard<std::mutex> lk(nd.mu);
+      auto it = nd.dev.find(in.opt.device);
+      if (it == nd.dev.end()) {
+        if (hipSetDevice(in.opt.device) != hipSuccess) return RISM3D_NO_DEVICE;
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+          std::fprintf(stderr, "rism3d_residual_noalloc: hipGetSymbolAddress failed: %s\n",
+                       hipGetErrorString(e));
+          return RISM3D_NO_DEVICE;
+        }
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+          return RISM3D_NO_DEVICE;
+        it = nd.dev.emplace(in.opt.device, std::make_pair(p, st)).first;
+      }
+      base = it->second.first;
+      in.stream = it->second.second;
+    }
+    rism::Arena arena;
+    arena.base = static_cast<char*>(base);
+    arena.size = cap;
+    rism::t_arena = &arena;
+    in.noalloc = 1;
+    return rism::run(in, &res, nullptr, nullptr);
+  } catch (const rism::ScratchFull&) {
+    std::fprintf(stderr, "rism3d_residual_noalloc: the fixed device scratch is full\n");
+    return RISM3D_NO_SCRATCH;
   } catch (const rism::HipFailure& e) {
     return (e.err == hipErrorOutOfMemory) ? RISM3D_OOM : RISM3D_NO_DEVICE;
   }

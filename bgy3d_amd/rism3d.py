@@ -27,8 +27,9 @@ LIBRARY = os.path.join(_HERE, "librism3d.so")
 
 CLOSURES = {"HNC": 0, "KH": 1, "PY": 2, **{f"PSE{n}": 3 + n for n in range(8)}}
 
-OK, NONCONVERGED, NAN, BAD_ARG, NO_DEVICE, OOM, UNSUPPORTED = range(7)
-STATUS_NAMES = ["OK", "NONCONVERGED", "NAN", "BAD_ARG", "NO_DEVICE", "OOM", "UNSUPPORTED"]
+OK, NONCONVERGED, NAN, BAD_ARG, NO_DEVICE, OOM, UNSUPPORTED, NO_SCRATCH = range(8)
+STATUS_NAMES = ["OK", "NONCONVERGED", "NAN", "BAD_ARG", "NO_DEVICE", "OOM", "UNSUPPORTED",
+                "NO_SCRATCH"]
 
 SOLVER_MDIIS, SOLVER_PICARD, SOLVER_NEWTON, SOLVER_JAGER, SOLVER_TRIAL = 0, 1, 2, 3, 4
 FP64, MIXED, FP32 = 0, 1, 2
@@ -133,6 +134,8 @@ def load():
         lib.rism3d_residual.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_int,
                                         P(Site), dp, P(Options), dp, dp]
         lib.rism3d_residual.restype = ctypes.c_int
+        lib.rism3d_residual_noalloc.argtypes = lib.rism3d_residual.argtypes
+        lib.rism3d_residual_noalloc.restype = ctypes.c_int
         lib.rism3d_jacobian.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_int,
                                         P(Site), dp, P(Options), dp, dp, ctypes.c_int, dp]
         lib.rism3d_jacobian.restype = ctypes.c_int
@@ -443,8 +446,10 @@ def batch_solve(solutes, solvent, settings, chi, want_g=False):
     return out
 
 
-def residual(solute, solvent, settings, chi, t):
-    """One evaluation of F(t) = (chi - 1) * c(t) - t (iterate_t1)."""
+def residual(solute, solvent, settings, chi, t, noalloc=False):
+    """One evaluation of F(t) = (chi - 1) * c(t) - t (iterate_t1).
+    noalloc=True calls rism3d_residual_noalloc (no device allocation, fixed
+    scratch of RISM3D_NOALLOC_SCRATCH_MB per device)."""
     lib = load()
     pd = problem_data(settings)
     opt = make_options(settings)
@@ -453,10 +458,11 @@ def residual(solute, solvent, settings, chi, t):
     t = np.ascontiguousarray(t, dtype=np.float64)
     r = np.empty_like(t)
     chi = np.ascontiguousarray(chi, dtype=np.float64)
-    st = lib.rism3d_residual(ctypes.byref(pd), len(sv), sv, len(su), su, _dptr(chi),
-                             ctypes.byref(opt), _dptr(t), _dptr(r))
+    fn = lib.rism3d_residual_noalloc if noalloc else lib.rism3d_residual
+    st = fn(ctypes.byref(pd), len(sv), sv, len(su), su, _dptr(chi), ctypes.byref(opt), _dptr(t),
+            _dptr(r))
     if st != OK:
-        raise RismError(st, "rism3d_residual")
+        raise RismError(st, fn.__name__)
     return r
 
 

@@ -75,7 +75,9 @@ enum rism3d_status {
   RISM3D_BAD_ARG = 3,
   RISM3D_NO_DEVICE = 4,
   RISM3D_OOM = 5,
-  RISM3D_UNSUPPORTED = 6    /* grid size or site count outside limits   */
+  RISM3D_UNSUPPORTED = 6,   /* grid size or site count outside limits   */
+  RISM3D_NO_SCRATCH = 7     /* rism3d_residual_noalloc: the problem needs
+                               more than the fixed device scratch         */
 };
 
 /* The reference's snes-solver choices (bgy3d_snes_default,
@@ -265,6 +267,33 @@ int rism3d_residual(const rism3d_problem *pd,
                     const double *chi_k,
                     const rism3d_options *opt,
                     const double *t, double *r);
+
+/* Capacity of the fixed device scratch of rism3d_residual_noalloc, per
+   device. */
+#define RISM3D_NOALLOC_SCRATCH_MB 96
+
+/* rism3d_residual without device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (serving
+   loops and harnesses that must not call hipMalloc / hipFree).  Same
+   arguments and result as rism3d_residual (t and r are host buffers,
+   copied through pageable transfers); options.coarse_tol is ignored.
+   Every device buffer of the call is taken from one fixed array of
+   RISM3D_NOALLOC_SCRATCH_MB MiB that each device holds from the library's
+   load until the process ends: the call makes no hipMalloc, hipFree or
+   pinned-host allocation, and replays no graph.  Its first call on a
+   device looks up that device's copy (hipGetSymbolAddress) and creates the
+   stream it keeps.  A problem that needs more scratch returns
+   RISM3D_NO_SCRATCH before any device work: 64^3 with 3 solvent sites
+   needs about 61 MB in FP64 and 82 MB mixed; 96^3 does not fit.  The
+   scratch keeps the last call's contents; each call writes what it reads.
+   Calls from several threads of one process share a device's scratch and
+   would race on it: the library's lock serialises them. */
+int rism3d_residual_noalloc(const rism3d_problem *pd,
+                            int m, const rism3d_site *solvent,
+                            int n, const rism3d_site *solute,
+                            const double *chi_k,
+                            const rism3d_options *opt,
+                            const double *t, double *r);
 
 /* One Jacobian application at t, replacing jacobian_t1
    (hnc3d.c:1290-1362): jdt = (chi - 1) * (dc/dt dt) - dt, or with
