@@ -8,12 +8,14 @@ chi(k) of the solvent is computed once and excluded from timing, as the
 reference memoises it (guile/bgy3d.scm:821).
 
 A step is one batched solve of all 64 solutes to convergence (setup of the
-solute fields, MDIIS iterations, mu, all on the GPU).  Mixed precision:
-FP32 FFT / closure / contraction until max|F| <= 1e-5, then 3 FP64 polish
-iterations.  One FP64 batch (max|F| <= 1e-10) is timed alongside for the
-speed ratio and |dmu|.  `natural_grids` reports the same 64 solutes on
-their own grid sizes (14 on 64^3, 49 on 96^3, 1 on 128^3), one batch per
-size.
+solute fields, MDIIS iterations, mu, PMV factor and g, all on the GPU; g
+stays in device memory).  Mixed precision: FP32 FFT / closure /
+contraction until max|F| <= 1e-5, then 3 FP64 polish iterations.  The FP64
+batch (max|F| <= 1e-10) is timed the same way (same warm-ups, same
+default path) for the speed ratio, |dmu| and max|dg|.  Per-kernel times
+come from one further mixed step with HIP events around every kernel.
+`natural_grids` reports the same 64 solutes on their own grid sizes (14 on
+64^3, 49 on 96^3, 1 on 128^3), one batch per size.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -70,31 +72,62 @@ def measured_traffic(workload, kernel):
     return rec.get("bytes_per_launch", {}).get(kernel)
 
 
-def cpu_baseline(solute, L_full, chi, fp64_iters, n_eval=4):
-    """The oracle (numpy FP64, FFTs threaded) on a bounded sample: setup of one
-    solute plus n_eval residual evaluations at 128^3, scaled to fp64_iters
-    residual evaluations (a cold-start FP64 solve of that solute)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(solute, L_full, chi, fp64_iters, budget_s=15.0):
+    """The oracle (numpy FP64) on a bounded sample of one solve: setup of
+    solute seed 0 at 128^3 plus as many residual evaluations of its cold
+    MDIIS iteration as fit in budget_s, scaled to fp64_iters evaluations (a
+    cold-start FP64 solve of that solute).  FFTs, the closure and the chi
+    contraction run on the box's CPU share (at most 16 threads)."""
     from oracle import closures as C
     from oracle import rism3d as O
     from bgy3d_amd.solutes import find_molecule
+    cores = max(1, min(16, os.cpu_count() or 1))
+    O.FFT_WORKERS = O.ELEMENTWISE_WORKERS = cores
     pd = O.Problem(128, L_full, 1.6889, 0.0333295, closure=C.KH, rmax=160.0, nrad=4096)
     t0 = time.perf_counter()
     s = O.Solute3D(pd, find_molecule(W), solute, chi)
     t_setup = time.perf_counter() - t0
-    t = np.zeros_like(s.v)
+    from oracle.nksolve import mdiis
+
+    class Budget(Exception):
+        pass
+
+    n_eval = 0
+
+    def F(x):
+        nonlocal n_eval
+        if n_eval >= 3 and time.perf_counter() - t0 >= budget_s:
+            raise Budget
+        n_eval += 1
+        return s.iterate(x)
+
     t0 = time.perf_counter()
-    for _ in range(n_eval):
-        r = s.iterate(t)
-        t = t + 0.1 * r
+    try:  # the solve's own MDIIS (depth 10, step 0.5) from t = 0, cut at the budget
+        mdiis(F, np.zeros_like(s.v), atol=1e-10, depth=10, lam=0.5, max_iter=100000)
+    except Budget:
+        pass
     t_eval = (time.perf_counter() - t0) / n_eval
     per_solve = t_setup + fp64_iters * t_eval
-    return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": O.FFT_WORKERS, "kind": "port",
-            "sample": (f"oracle/rism3d.py (numpy FP64; FFTs on {O.FFT_WORKERS} threads, the "
-                       f"rest on one) on solute seed 0 at "
-                       f"128^3: setup {t_setup:.2f} s + {n_eval} residual evaluations at "
-                       f"{t_eval:.2f} s each, scaled to {fp64_iters:.0f} evaluations "
-                       f"(the residual count of a cold-start FP64 MDIIS solve of the same "
-                       f"solute on the GPU)")}
+    return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": cores, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": (f"oracle/rism3d.py (numpy FP64 restatement of the reference algorithm; "
+                       f"scipy.fft FFTs, closure and chi contraction on {cores} threads) on "
+                       f"solute seed 0 at 128^3: setup {t_setup:.2f} s + the first {n_eval} "
+                       f"MDIIS residual evaluations from t = 0 at {t_eval:.3f} s each, "
+                       f"scaled to {fp64_iters:.0f} "
+                       f"evaluations (the residual count of a cold-start FP64 MDIIS solve of "
+                       f"the same solute on the GPU)")}
 
 
 def natural_grids(solutes, chi, args):
@@ -102,7 +135,7 @@ def natural_grids(solutes, chi, args):
     of 64^3 / 96^3 / 128^3 at <= 0.35 A spacing with 8 A padding), one
     batched launch sequence per grid size in a box that pads every member:
     solves/s over the whole set, mixed and FP64, and |dmu| between them.
-    One untimed and one timed pass."""
+    One untimed and one timed pass of each precision."""
     from bgy3d_amd import rism3d as R
     from bgy3d_amd import synthetic as S
     groups = {}
@@ -124,6 +157,7 @@ def natural_grids(solutes, chi, args):
         return out
 
     run("mixed")
+    run("fp64")
     t0 = time.perf_counter()
     mixed = run("mixed")
     t_mixed = time.perf_counter() - t0
@@ -137,21 +171,6 @@ def natural_grids(solutes, chi, args):
             "mixed_over_fp64": t_fp64 / t_mixed, "max_abs_dmu_kcal": max(dmu),
             "all_converged": all(a["status"] == "OK" and b["status"] == "OK"
                                  for a, b in zip(mixed, fp64))}
-
-
-def warm_device_memory(gb=96):
-    """Map and release gb GB of device memory once, before the solver's own
-    buffers and outside every timed region.  On a freshly acquired MI355X
-    the first process to allocate this much runs its streaming kernels
-    about 4% slower for its whole life (closure 3108 against 2900 us per
-    launch at 64 x 128^3); after one allocate-and-free of this size the
-    first bench process runs at the speed of later ones (48.0 against
-    46.8 solves/s)."""
-    import torch
-    x = [torch.ones(int(8e9), dtype=torch.uint8, device="cuda") for _ in range(gb // 8)]
-    torch.cuda.synchronize()
-    del x
-    torch.cuda.empty_cache()
 
 
 def main():
@@ -185,39 +204,66 @@ def main():
               "but multi-GPU runs are not part of it.", file=sys.stderr)
         sys.exit(2)
 
+    # The library keeps every buffer it releases (up to 192 GB) instead of
+    # only the last call's: the bench alternates precisions and grid sizes.
+    os.environ.setdefault("RISM3D_CACHE_MB", "196608")
+    # g of every solute is written into device buffers held by torch; torch
+    # loads the HIP runtime before the library, which then uses the same one
+    import torch
+    torch.cuda.init()
+    dev = torch.device("cuda", 0)
+
     from bgy3d_amd import rism3d as R
     from bgy3d_amd import synthetic as S
 
     if args.calibrate_copy:
-        import torch
         n = 64 * 1024 * 1024  # doubles: 512 MiB
-        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        a = torch.ones(n, dtype=torch.float64, device=dev)
         b = torch.empty_like(a)
         for _ in range(3):
             torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
         del a, b
 
-    warm_device_memory()
     solutes = S.batch(range(args.nsolutes))
+    n = len(solutes)
     L_full = S.forced_box(solutes, N=128)
     chi = chi_solvent()
     mixed = settings_for(L_full, "mixed", args.diis_depth, args.diis_step, args.polish,
                          args.mixed_tol, coarse_tol=args.coarse_tol)
     fp64 = settings_for(L_full, "fp64", args.diis_depth, args.diis_step,
                         coarse_tol=args.coarse_tol)
+    g_mixed = torch.empty((n, 3, 128, 128, 128), dtype=torch.float64, device=dev)
+    ptr_mixed = [g_mixed[b].data_ptr() for b in range(n)]
+    g_fp64 = None if args.skip_fp64 else torch.empty_like(g_mixed)
+    ptr_fp64 = None if args.skip_fp64 else [g_fp64[b].data_ptr() for b in range(n)]
+    torch.cuda.synchronize()
 
+    # equal warm-ups, then each precision timed on the library's default path
     for _ in range(args.warmup):
-        R.batch_solve(solutes, W, mixed, chi)
+        R.batch_solve(solutes, W, mixed, chi, g_device=ptr_mixed)
+    if not args.skip_fp64:
+        for _ in range(args.warmup):
+            R.batch_solve(solutes, W, fp64, chi, g_device=ptr_fp64)
 
-    R.profile_start(0)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res_mixed = R.batch_solve(solutes, W, mixed, chi)
+        res_mixed = R.batch_solve(solutes, W, mixed, chi, g_device=ptr_mixed)
     t_mixed = (time.perf_counter() - t0) / args.steps
+    if not args.skip_fp64:
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res_fp64 = R.batch_solve(solutes, W, fp64, chi, g_device=ptr_fp64)
+        t_fp64 = (time.perf_counter() - t0) / args.steps
+
+    # per-kernel HIP-event timing in a separate pass of the same mixed batch
+    # (events bracket each kernel on the solver's stream, launched one by one)
+    R.profile_start(0)
+    t0 = time.perf_counter()
+    R.batch_solve(solutes, W, mixed, chi, g_device=ptr_mixed)
+    t_prof = time.perf_counter() - t0
     prof = R.profile_read()
 
-    n = len(solutes)
     value = n / t_mixed
     dom = max(prof, key=lambda k: prof[k][0])
     ms, by, nl = prof[dom]
@@ -239,13 +285,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "grid": 128, "box_A": L_full, "solutes": n,
                    "closure": "KH", "mixed_tol": args.mixed_tol, "polish_iters": args.polish,
-                   "fp64_tol": 1e-10, "coarse_tol": args.coarse_tol},
+                   "fp64_tol": 1e-10, "coarse_tol": args.coarse_tol,
+                   "outputs": "mu, PMV factor and g of every solute (g on the device)"},
     }
     fp64_iters = None
     if not args.skip_fp64:
-        t0 = time.perf_counter()
-        res_fp64 = R.batch_solve(solutes, W, fp64, chi)
-        t_fp64 = time.perf_counter() - t0
         ok = [a["status"] == "OK" and b["status"] == "OK" for a, b in zip(res_mixed, res_fp64)]
         dmu = [abs(a["free-energy"] - b["free-energy"]) for a, b in zip(res_mixed, res_fp64)]
         worst = int(np.argmax(dmu))
@@ -255,6 +299,7 @@ def main():
             "fp64_solves_per_s": fp64_rate,
             "mixed_over_fp64": value / fp64_rate,
             "max_abs_dmu_kcal": max(dmu),
+            "max_abs_dg": float((g_mixed - g_fp64).abs().max()),
             "all_converged": all(ok),
             "mean_iters_fp64": fp64_iters,
             "worst_solute": {"seed": worst, "dmu": dmu[worst],
@@ -270,7 +315,10 @@ def main():
         "diis": {"depth": args.diis_depth, "step": args.diis_step, "polish": args.polish},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload, dom),
-                     "kernel": dom, "avg_launch_us": ms / nl * 1e3, "bytes_per_launch": by / nl},
+                     "kernel": dom, "avg_launch_us": ms / nl * 1e3, "bytes_per_launch": by / nl,
+                     "timing": ("HIP events around each launch on the solver's stream, in one "
+                                "profiled mixed step after the timed steps"),
+                     "profiled_step_ms": t_prof * 1e3},
         "kernels": {k: {"avg_us": v[0] / max(v[2], 1) * 1e3,
                         "GB_s": (v[1] / (v[0] * 1e-3) / 1e9) if v[0] > 0 else None,
                         "bytes_per_launch": v[1] / max(v[2], 1),

@@ -206,14 +206,21 @@ void launch_zstar(int N2, int NX, int N1, int B, hipStream_t s, const SoluteDesc
 // first (hipMalloc / hipFree cost milliseconds per large buffer and
 // synchronise the device).  Every solve runs on one stream and synchronises
 // it before returning, so a released buffer is idle when handed out again.
-// RISM3D_CACHE_MB caps the cache (default 131072 MB, 0 disables it);
+// By default the cache keeps only what the last call used: at the end of
+// each call the buffers it did not take are freed, so a change of shape or
+// precision returns the memory of the old one.  RISM3D_CACHE_MB instead
+// keeps every released buffer up to that many MB (0 disables the cache);
 // rism3d_cache_release() empties it.  On an allocation failure the cache of
 // that device is emptied and the allocation retried.
 struct MemCache {
   std::mutex mu;
-  std::multimap<std::pair<int, size_t>, void*> blocks;
+  // (device, bytes) -> (buffer, call that released it)
+  std::multimap<std::pair<int, size_t>, std::pair<void*, uint64_t>> blocks;
   size_t bytes = 0;
-  size_t limit = (size_t)std::max(0, env_int("RISM3D_CACHE_MB", 131072)) << 20;
+  uint64_t call = 0;  // calls so far
+  bool keep_last_only = std::getenv("RISM3D_CACHE_MB") == nullptr;
+  size_t limit = keep_last_only ? SIZE_MAX
+                                : (size_t)std::max(0, env_int("RISM3D_CACHE_MB", 0)) << 20;
 };
 MemCache& mem_cache() {
   static MemCache c;
@@ -227,11 +234,30 @@ void cache_empty(int dev) {  // dev < 0: every device
   for (auto it = c.blocks.begin(); it != c.blocks.end();) {
     if (dev >= 0 && it->first.first != dev) { ++it; continue; }
     (void)hipSetDevice(it->first.first);
-    (void)hipFree(it->second);
+    (void)hipFree(it->second.first);
     c.bytes -= it->first.second;
     it = c.blocks.erase(it);
   }
   (void)hipSetDevice(cur);
+}
+// End of a call: unless RISM3D_CACHE_MB asks to keep more, free the cached
+// buffers this call did not release (those of earlier calls' shapes).
+void cache_end_call() {
+  MemCache& c = mem_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (c.keep_last_only) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto it = c.blocks.begin(); it != c.blocks.end();) {
+      if (it->second.second == c.call) { ++it; continue; }
+      (void)hipSetDevice(it->first.first);
+      (void)hipFree(it->second.first);
+      c.bytes -= it->first.second;
+      it = c.blocks.erase(it);
+    }
+    (void)hipSetDevice(cur);
+  }
+  ++c.call;
 }
 // A bump arena over memory the library already holds (the fixed scratch).
 // While a thread has one installed, dev_alloc carves its buffers from it and
@@ -259,7 +285,7 @@ void* dev_alloc(size_t bytes) {
     std::lock_guard<std::mutex> lk(c.mu);
     auto it = c.blocks.find({dev, bytes});
     if (it != c.blocks.end()) {
-      void* p = it->second;
+      void* p = it->second.first;
       c.blocks.erase(it);
       c.bytes -= bytes;
       return p;
@@ -282,7 +308,7 @@ void dev_free(void* p, size_t bytes) {
   {
     std::lock_guard<std::mutex> lk(c.mu);
     if (c.bytes + bytes <= c.limit) {
-      c.blocks.insert({{dev, bytes}, p});
+      c.blocks.insert({{dev, bytes}, {p, c.call}});
       c.bytes += bytes;
       return;
     }
@@ -1686,6 +1712,9 @@ int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt.device) return RISM3D_NO_DEVICE;
   std::lock_guard<std::mutex> lock(g_mutex);
+  struct EndCall {
+    ~EndCall() { cache_end_call(); }
+  } ec;
   HIPCHK(hipSetDevice(opt.device));
   Problem pb;
   pb.B = 1;
@@ -2214,7 +2243,10 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     bool want_g = false;
     for (int b = 0; b < B; ++b) {
       tt[b] = pb.st[b].t64.p;
-      if (res[b].g) {
+      if (res[b].g_device) {  // the caller's device buffer
+        gg[b] = res[b].g_device;
+        want_g = true;
+      } else if (res[b].g) {
         gbuf[b] = DevBuf<double>(pb.mP);
         gg[b] = gbuf[b].p;
         want_g = true;
@@ -2258,7 +2290,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
       res[b].iterations_fp32 = s.iters32;
       res[b].status = s.status;
       if (res[b].g)
-        HIPCHK(hipMemcpy(res[b].g, gbuf[b].p, pb.mP * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(res[b].g, gg[b], pb.mP * 8, hipMemcpyDeviceToHost));
       if (res[b].t)
         HIPCHK(hipMemcpy(res[b].t, tt[b], pb.mP * 8, hipMemcpyDeviceToHost));
       if (t_final && b == 0) {
@@ -2309,6 +2341,12 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
 int run(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
         std::vector<double>* t_final) {
   std::lock_guard<std::mutex> lock(g_mutex);
+  struct EndCall {  // an allocation-free call leaves the cache alone
+    bool on;
+    ~EndCall() {
+      if (on) cache_end_call();
+    }
+  } ec{!in.noalloc};
   return run_impl(in, res, t_init, t_final, nullptr);
 }
 
@@ -2619,6 +2657,7 @@ int rism3d_cache_release(void) {
 const char* rism3d_version(void) { return "rism3d-mi355x 0.1 (gfx950)"; }
 
 int rism3d_profile_start(int device) {
+  std::lock_guard<std::mutex> lock(rism::g_mutex);
   if (hipSetDevice(device) != hipSuccess) return RISM3D_NO_DEVICE;
   auto& pf = rism::g_prof;
   for (int k = 0; k < 5; ++k)
@@ -2634,6 +2673,7 @@ int rism3d_profile_start(int device) {
 }
 
 int rism3d_profile_read(double* ms, double* bytes, long* launches) {
+  std::lock_guard<std::mutex> lock(rism::g_mutex);
   auto& pf = rism::g_prof;
   for (int k = 0; k < 10; ++k) {
     if (ms) ms[k] = pf.ms[k];

@@ -83,7 +83,8 @@ class Result(ctypes.Structure):
                 ("dmu_dsite", ctypes.POINTER(ctypes.c_double)),
                 ("adjoint_iterations", ctypes.c_int),
                 ("response", ctypes.POINTER(ctypes.c_double)),
-                ("info", ctypes.c_void_p)]
+                ("info", ctypes.c_void_p),
+                ("g_device", ctypes.c_void_p)]
 
 
 class Info(ctypes.Structure):
@@ -412,8 +413,11 @@ def restart_destroy(token):
         load().rism3d_restart_destroy(ctypes.c_void_p(token))
 
 
-def batch_solve(solutes, solvent, settings, chi, want_g=False):
-    """Many solutes, one grid and solvent: rism3d_batch_solve."""
+def batch_solve(solutes, solvent, settings, chi, want_g=False, g_device=None):
+    """Many solutes, one grid and solvent: rism3d_batch_solve.
+    g_device: None, or one device address per solute (a buffer of
+    m N2 N1 N0 doubles on settings' device each, e.g. a torch tensor's
+    data_ptr()) that receives g on the device."""
     lib = load()
     pd = problem_data(settings)
     opt = make_options(settings)
@@ -430,6 +434,11 @@ def batch_solve(solutes, solvent, settings, chi, want_g=False):
             g = np.empty((m, pd.N[2], pd.N[1], pd.N[0]))
             gs.append(g)
             res[b].g = _dptr(g)
+    if g_device is not None:
+        if len(g_device) != B:
+            raise ValueError("g_device needs one device address per solute")
+        for b in range(B):
+            res[b].g_device = ctypes.c_void_p(int(g_device[b]))
     chi = np.ascontiguousarray(chi, dtype=np.float64)
     st = lib.rism3d_batch_solve(ctypes.byref(pd), m, sv, B, ns, ptrs, _dptr(chi),
                                 ctypes.byref(opt), res)

@@ -195,6 +195,10 @@ typedef struct rism3d_result {
                           (closures.f90:131-153)                          */
   struct rism3d_info *info;  /* in: NULL or caller struct; out: the
                           observables print_info prints (below)           */
+  double *g_device;    /* in: NULL or a device buffer [m][N2][N1][N0] on
+                          options.device; out: g written there on the
+                          device (no host transfer; g above, if also
+                          given, receives a copy)                         */
 } rism3d_result;
 
 /* The post-solve observables of the solute in its solvent: info() and
@@ -368,10 +372,14 @@ int rism3d_solvent_chi(const rism3d_problem *pd,
 int rism3d_profile_start(int device);
 int rism3d_profile_read(double *ms, double *bytes, long *launches);
 
-/* Device memory is cached across solves (buffers of the same size are
-   reused instead of freed; RISM3D_CACHE_MB caps the cache, default
-   131072).  This frees every cached buffer on every device.  No
-   counterpart in the reference, which allocates and frees per solve. */
+/* Device memory is cached across calls: a call's buffers are kept when it
+   returns and reused by the next call of the same shape; at the end of
+   each call the buffers it did not use are freed, so the cache holds at
+   most the last call's working set.  With the environment variable
+   RISM3D_CACHE_MB set, every released buffer is kept instead, up to that
+   many MB (0 disables the cache).  This frees every cached buffer on every
+   device.  No counterpart in the reference, which allocates and frees per
+   solve. */
 int rism3d_cache_release(void);
 
 /* Library build/version string, for diagnostics. */

@@ -245,6 +245,32 @@ def solvent_kernel_rism(pd, solvent, chi_1d):
 # pocketfft through scipy.fft on up to 16 host threads (the same transforms
 # as numpy.fft, which runs them on one thread)
 FFT_WORKERS = max(1, min(16, os.cpu_count() or 1))
+# The closure and the chi contraction split over slabs of z planes on this
+# many threads (numpy releases the GIL inside its loops); 1 runs them on
+# the calling thread.  Same arithmetic either way, point by point.
+ELEMENTWISE_WORKERS = 1
+_POOL = None
+
+
+def _zsplit(fn, zaxes, *arrays):
+    """fn(*slabs) over slabs of z planes, joined along the output's z axis
+    (axis 1 of an [m][N2][N1][N0] result); zaxes[i] is the z axis of
+    arrays[i]."""
+    global _POOL
+    nz = arrays[0].shape[zaxes[0]]
+    nw = min(ELEMENTWISE_WORKERS, nz)
+    if nw <= 1:
+        return fn(*arrays)
+    if _POOL is None or _POOL._max_workers < nw:
+        from concurrent.futures import ThreadPoolExecutor
+        _POOL = ThreadPoolExecutor(max_workers=nw)
+    cuts = np.linspace(0, nz, nw + 1).astype(int)
+
+    def part(k):
+        z = slice(cuts[k], cuts[k + 1])
+        return fn(*[a[(slice(None),) * ax + (z,)] for a, ax in zip(arrays, zaxes)])
+
+    return np.concatenate(list(_POOL.map(part, range(nw))), axis=1)
 
 
 def rfft(pd, a):
@@ -310,10 +336,12 @@ class Solute3D:
         self.v = np.stack([field0(pd, a, solute) for a in solvent])
 
     def _closure(self, t):
-        return C.closure(self.pd.closure, self.pd.beta, self.v, t)
+        return _zsplit(lambda v, tt: C.closure(self.pd.closure, self.pd.beta, v, tt), (1, 1),
+                       self.v, t)
 
     def star(self, x):
-        return np.einsum("ijzyx,jzyx->izyx", self.chi, x)
+        return _zsplit(lambda chi, xx: np.einsum("ijzyx,jzyx->izyx", chi, xx), (2, 1),
+                       self.chi, x)
 
     def iterate(self, t):
         """F(t) = T(t) - t (hnc3d.c:1184-1285)."""
