@@ -208,4 +208,59 @@ __device__ __forceinline__ void fft_lines_il(cpx<T>* buf, int nl, const cpx<T>* 
   }
 }
 
+// The FFT of the NCOL columns of a row-major plane in LDS: point p of
+// column l at buf[p * LS + l], any row stride LS >= NCOL.  Butterflies are
+// assigned column-fastest, so consecutive lanes touch consecutive LDS words
+// (the layout of stockham_stage_il without its power-of-two line count).
+template <typename T, int N, int LS, int NCOL, int NT, int STAGE, bool INV>
+__device__ __forceinline__ void stockham_stage_cols(cpx<T>* buf, const cpx<T>* __restrict__ tw) {
+  constexpr int R = radix_at(N, STAGE);
+  constexpr int NS = stage_ns(N, STAGE);
+  constexpr int NB = N / R;
+  constexpr int BF = NCOL * NB;
+  constexpr int PER = (BF + NT - 1) / NT;
+  constexpr int TWS = N / (NS * R);
+  cpx<T> v[PER][4];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = tid + p * NT;
+    if (PER * NT == BF || b < BF) {
+      const int col = b % NCOL, j = b / NCOL;
+      const int k = j % NS;
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[p][q] = buf[(j + q * NB) * LS + col];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          const cpx<T> w = tw[q * k * TWS];
+          v[p][q] = INV ? cmulc(v[p][q], w) : cmul(v[p][q], w);
+        }
+      }
+      dft_small<T, R, INV>(v[p]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = tid + p * NT;
+    if (PER * NT == BF || b < BF) {
+      const int col = b % NCOL, j = b / NCOL;
+      const int k = j % NS;
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[(base + q * NS) * LS + col] = v[p][q];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int LS, int NCOL, int NT, bool INV, int S = 0>
+__device__ __forceinline__ void fft_cols(cpx<T>* buf, const cpx<T>* __restrict__ tw) {
+  if constexpr (S < num_stages(N)) {
+    stockham_stage_cols<T, N, LS, NCOL, NT, S, INV>(buf, tw);
+    fft_cols<T, N, LS, NCOL, NT, INV, S + 1>(buf, tw);
+  }
+}
+
 }  // namespace rism

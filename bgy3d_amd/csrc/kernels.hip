@@ -130,6 +130,63 @@ __device__ __forceinline__ void stbf4(uint16_t* p, BF4 q) {
   *(G*)p = x;
 }
 
+// One vector of VW points at e (an offset into the slab off of solute d's
+// site arrays): the MDIIS step t = sum_j coef_j u_j (written to t) or the
+// given t, then c = closure(beta v, t) in the solver mode of d.
+template <typename T, int H>
+__device__ __forceinline__ Vec16<T> closure_vec(const SoluteDesc& d, const GridArgs& g,
+                                                int64_t off, int e, T* __restrict__ t,
+                                                const T* __restrict__ bv) {
+  constexpr int VW = Vec16<T>::W;
+  const int nc = d.ncomb;
+  Vec16<T> c;
+  Vec16<T> tv;
+  const Vec16<T> b = ld16(bv + e);
+  if (nc > 0) {
+    // all loads first, then the sums: independent loads in flight
+    Vec16<T> u[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+      if (j < nc) u[j] = ld16(static_cast<const T*>(d.u[j]) + off + e);
+    Vec16<T> sh;
+    if (d.shift_w != 0.0) sh = ld16(static_cast<const T*>(d.shift) + off + e);
+    double acc[VW];
+#pragma unroll
+    for (int k = 0; k < VW; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+      if (j < nc) {
+        const double cj = d.coef[j];
+#pragma unroll
+        for (int k = 0; k < VW; ++k) acc[k] += cj * (double)u[j].v[k];
+      }
+    if (d.shift_w != 0.0) {
+#pragma unroll
+      for (int k = 0; k < VW; ++k) acc[k] -= d.shift_w * (double)sh.v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < VW; ++k) tv.v[k] = T(acc[k]);
+    st16(t + e, tv);
+  } else {
+    tv = ld16(t + e);
+  }
+#pragma unroll
+  for (int k = 0; k < VW; ++k) {
+    if (d.mode == 0) c.v[k] = closure_c<T>(g.closure, g.pse_order, b.v[k], tv.v[k]);
+    else if (d.mode == 5)  // bv holds x_b (PY: exp(-beta v) - 1, c linear in t)
+      c.v[k] = (g.closure == 2) ? b.v[k] * tv.v[k]
+                                : closure_dc<T>(g.closure, g.pse_order, b.v[k], tv.v[k]);
+    else if (d.mode == 6)  // J v: dc = (dc/dt) v (compute_c1, hnc3d.c:1307)
+      c.v[k] = T(gld(d.dt + off + e + k) * (double)tv.v[k]);
+    else if (d.mode == 7)  // flag form: dc + v (hnc3d.c:1310-1320)
+      c.v[k] = T((gld(d.dt + off + e + k) + 1.0) * (double)tv.v[k]);
+    else if (d.mode == 10)  // rho h for the susceptibility (hnc3d.c:1006-1030)
+      c.v[k] = T(g.rho) * (closure_c<T>(g.closure, g.pse_order, b.v[k], tv.v[k]) + tv.v[k]);
+    else c.v[k] = tv.v[k];
+  }
+  return c;
+}
+
 // ------------------------------------------- 1. closure fused with x r2c
 // t = sum_j coef_j u_j (the MDIIS step; written back as the iterate) or the
 // given t, then c = closure(beta v, t), then the x-axis r2c of c.
@@ -150,7 +207,6 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
   const int64_t off = site * g.P + (int64_t)row0 * N0;
   T* __restrict__ t = static_cast<T*>(d.t) + off;
   const T* __restrict__ bv = static_cast<const T*>(d.bv) + off;
-  const int nc = d.ncomb;
   const int vrows = min(ROWS, nrows - row0);
   const int nvalid = vrows * N0;
   constexpr int VW = Vec16<T>::W;
@@ -158,52 +214,7 @@ closure_r2c_x(const SoluteDesc* __restrict__ sd, GridArgs g,
     Vec16<T> c;
 #pragma unroll
     for (int k = 0; k < VW; ++k) c.v[k] = T(0);
-    if (e < nvalid) {  // nvalid is a multiple of N0: whole vectors
-      Vec16<T> tv;
-      const Vec16<T> b = ld16(bv + e);
-      if (nc > 0) {
-        // all loads first, then the sums: independent loads in flight
-        Vec16<T> u[H];
-#pragma unroll
-        for (int j = 0; j < H; ++j)
-          if (j < nc) u[j] = ld16(static_cast<const T*>(d.u[j]) + off + e);
-        Vec16<T> sh;
-        if (d.shift_w != 0.0) sh = ld16(static_cast<const T*>(d.shift) + off + e);
-        double acc[VW];
-#pragma unroll
-        for (int k = 0; k < VW; ++k) acc[k] = 0.0;
-#pragma unroll
-        for (int j = 0; j < H; ++j)
-          if (j < nc) {
-            const double cj = d.coef[j];
-#pragma unroll
-            for (int k = 0; k < VW; ++k) acc[k] += cj * (double)u[j].v[k];
-          }
-        if (d.shift_w != 0.0) {
-#pragma unroll
-          for (int k = 0; k < VW; ++k) acc[k] -= d.shift_w * (double)sh.v[k];
-        }
-#pragma unroll
-        for (int k = 0; k < VW; ++k) tv.v[k] = T(acc[k]);
-        st16(t + e, tv);
-      } else {
-        tv = ld16(t + e);
-      }
-#pragma unroll
-      for (int k = 0; k < VW; ++k) {
-        if (d.mode == 0) c.v[k] = closure_c<T>(g.closure, g.pse_order, b.v[k], tv.v[k]);
-        else if (d.mode == 5)  // bv holds x_b (PY: exp(-beta v) - 1, c linear in t)
-          c.v[k] = (g.closure == 2) ? b.v[k] * tv.v[k]
-                                    : closure_dc<T>(g.closure, g.pse_order, b.v[k], tv.v[k]);
-        else if (d.mode == 6)  // J v: dc = (dc/dt) v (compute_c1, hnc3d.c:1307)
-          c.v[k] = T(gld(d.dt + off + e + k) * (double)tv.v[k]);
-        else if (d.mode == 7)  // flag form: dc + v (hnc3d.c:1310-1320)
-          c.v[k] = T((gld(d.dt + off + e + k) + 1.0) * (double)tv.v[k]);
-        else if (d.mode == 10)  // rho h for the susceptibility (hnc3d.c:1006-1030)
-          c.v[k] = T(g.rho) * (closure_c<T>(g.closure, g.pse_order, b.v[k], tv.v[k]) + tv.v[k]);
-        else c.v[k] = tv.v[k];
-      }
-    }
+    if (e < nvalid) c = closure_vec<T, H>(d, g, off, e, t, bv);  // whole vectors
     // z_n = c_{2n} + i c_{2n+1} of row e / N0
     cpx<T>* zrow = lds + (e / N0) * LSM;
     const int n0 = (e % N0) / 2;
@@ -947,6 +958,285 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
     const int q = threadIdx.x;
     double a = red[q][0];
     for (int i = 1; i < NT / 64; ++i) a = (q == 0) ? fmax(a, red[q][i]) : a + red[q][i];
+    gst(d.part + ((int64_t)site * gridDim.x + blockIdx.x) * NPART + q, a);
+  }
+}
+
+// ------------------------------- 1' and 5'. the xy-plane pair (FP32)
+// The FP32 residual in three launches: closure_r2c_xy, zstar, c2r_xy_resid.
+// A workgroup owns one z plane of one site of one solute: N1 rows of N0
+// points, whose half spectrum (N1 rows of N0/2 + 1 bins, 66.5 KB in FP32
+// at 128^2) stays in LDS between the x and the y transforms, so the y pass
+// of the five-launch chain costs no HBM traffic.  Rows of the plane are at
+// stride LS = N0/2 + 1 in LDS; the row FFTs run on the first N0/2 points of
+// each row, the column FFTs on all N0/2 + 1 columns.
+constexpr int NTP = 512;  // threads of a plane workgroup
+
+// t, c = closure(beta v, t) as closure_r2c_x over the plane, the N0-point
+// r2c of every row (N0/2-point FFT of z_n = c_2n + i c_2n+1 and the split,
+// in place: bins k and N0/2 - k from Z_k and Z_{N0/2-k}), then the c2c of
+// every kx column along y; the plane's rows [y][kx] go to W.
+// Grid: x = z plane (N2), y = site, z = solute.
+template <typename T, int N0, int N1, int H>
+__global__ void __launch_bounds__(NTP)
+closure_r2c_xy(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw_half,
+               const cpx<T>* __restrict__ tw_full, const cpx<T>* __restrict__ tw_y) {
+  constexpr int M = N0 / 2, LS = M + 1, PLANE = N0 * N1, HP = M / 2 + 1;
+  constexpr int VW = Vec16<T>::W;
+  static_assert(PLANE % VW == 0, "whole vectors");
+  constexpr bool WHOLE = PLANE % (2 * NTP * VW) == 0;  // batches of two vectors
+  __shared__ cpx<T> lds[N1 * LS];
+  const SoluteDesc& d = sd[blockIdx.z];
+  if (!d.active) return;
+  const int site = blockIdx.y, z = blockIdx.x;
+  const int64_t off = site * g.P + (int64_t)z * PLANE;
+  T* __restrict__ t = static_cast<T*>(d.t) + off;
+  const T* __restrict__ bv = static_cast<const T*>(d.bv) + off;
+  // two vectors per step, so that both vectors' streams are in flight together
+  for (int e0 = threadIdx.x * VW; e0 < PLANE; e0 += 2 * NTP * VW) {
+    Vec16<T> c[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (WHOLE || e0 + q * NTP * VW < PLANE)
+        c[q] = closure_vec<T, H>(d, g, off, e0 + q * NTP * VW, t, bv);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = e0 + q * NTP * VW;
+      if (!WHOLE && e >= PLANE) continue;
+      cpx<T>* zrow = lds + (e / N0) * LS;
+      const int n0 = (e % N0) / 2;
+#pragma unroll
+      for (int h = 0; h < VW / 2; ++h) zrow[n0 + h] = cpx<T>{c[q].v[2 * h], c[q].v[2 * h + 1]};
+    }
+  }
+  __syncthreads();
+  fft_lines<T, M, N1, NTP, false>(lds, LS, N1, tw_half);
+  for (int e = threadIdx.x; e < N1 * HP; e += NTP) {
+    const int row = e / HP, k = e - row * HP;
+    cpx<T>* r = lds + row * LS;
+    const cpx<T> a = r[k], b = r[k == 0 ? 0 : M - k];  // Z_k, Z_{M-k}
+    // bin kk from Z_kk and Z_{M-kk}: (Z + conj Z') / 2 - i w_kk (Z - conj Z') / 2
+    auto bin = [&](int kk, cpx<T> zk, cpx<T> zo) {
+      const cpx<T> zm = cconj(zo);
+      const cpx<T> sm = cadd(zk, zm);
+      const cpx<T> dd = rot90<T, false>(cmul(tw_full[kk], csub(zk, zm)));
+      return cpx<T>{T(0.5) * (sm.re + dd.re), T(0.5) * (sm.im + dd.im)};
+    };
+    if (k == 0) {
+      r[0] = bin(0, a, a);
+      r[M] = bin(M, a, a);
+    } else {
+      const cpx<T> xk = bin(k, a, b), xm = bin(M - k, b, a);
+      r[k] = xk;
+      r[M - k] = xm;
+    }
+  }
+  __syncthreads();
+  fft_cols<T, N1, LS, M + 1, NTP, false>(lds, tw_y);
+  cpx<T>* W = static_cast<cpx<T>*>(d.W) + site * g.K + (int64_t)z * N1 * g.NH;
+  for (int e = threadIdx.x; e < N1 * (M + 1); e += NTP) {
+    const int y = e / (M + 1), k = e - y * (M + 1);
+    stc(W + y * g.NH + k, lds[y * LS + k]);
+  }
+}
+
+// The inverse: the plane's rows [y][kx] from W, the inverse c2c of every
+// column along y, the c2r pre-processing of every row in place (Z_k and
+// Z_{N0/2-k} from bins k and N0/2 - k; Im of the DC and Nyquist bins
+// dropped, as FFTW's c2r does), the inverse N0/2-point row FFTs, then
+// r = t' - t (or the mode's form), u = t + lam r and the partials max|r|,
+// sum r^2, <r, r_j> as c2r_x_resid, one partial row per plane at
+// part[(site * N2 + z) * NPART + q].  Grid as closure_r2c_xy.
+template <typename T, int N0, int N1, int H>
+__global__ void __launch_bounds__(NTP)
+c2r_xy_resid(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw_half,
+             const cpx<T>* __restrict__ tw_full, const cpx<T>* __restrict__ tw_y) {
+  constexpr int M = N0 / 2, LS = M + 1, PLANE = N0 * N1, HP = M / 2 + 1;
+  constexpr int VW = Vec16<T>::W;
+  static_assert(PLANE % VW == 0, "whole vectors");
+  constexpr bool WHOLE = PLANE % (2 * NTP * VW) == 0;  // batches of two vectors
+  __shared__ cpx<T> lds[N1 * LS];
+  __shared__ double red[NPART][NTP / 64];
+  const SoluteDesc& d = sd[blockIdx.z];
+  if (!d.active) return;
+  const int site = blockIdx.y, z = blockIdx.x;
+  const cpx<T>* W = static_cast<const cpx<T>*>(d.W) + site * g.K + (int64_t)z * N1 * g.NH;
+  {
+    constexpr int PTS = N1 * (M + 1), PT = (PTS + NTP - 1) / NTP;
+    cpx<T> v[PT];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {  // all loads first
+      const int e = threadIdx.x + p * NTP;
+      if (PT * NTP == PTS || e < PTS) {
+        const int y = e / (M + 1), k = e - y * (M + 1);
+        v[p] = ldc(W + y * g.NH + k);
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      const int e = threadIdx.x + p * NTP;
+      if (PT * NTP == PTS || e < PTS) {
+        const int y = e / (M + 1), k = e - y * (M + 1);
+        lds[y * LS + k] = v[p];
+      }
+    }
+  }
+  __syncthreads();
+  fft_cols<T, N1, LS, M + 1, NTP, true>(lds, tw_y);
+  for (int e = threadIdx.x; e < N1 * HP; e += NTP) {
+    const int row = e / HP, k = e - row * HP;
+    cpx<T>* r = lds + row * LS;
+    cpx<T> xk = r[k], xm = r[M - k];  // bins k and M - k (0: DC and Nyquist)
+    // Z_kk = (X_kk + conj X_{M-kk}) + i e^{+2 pi i kk/N0} (X_kk - conj X_{M-kk})
+    auto pre = [&](int kk, cpx<T> a0, cpx<T> b0) {
+      const cpx<T> a = cadd(a0, cconj(b0));
+      const cpx<T> b = csub(a0, cconj(b0));
+      return cadd(a, rot90<T, true>(cmulc(b, tw_full[kk])));
+    };
+    if (k == 0) {
+      xk.im = T(0);
+      xm.im = T(0);
+      r[0] = pre(0, xk, xm);
+    } else {
+      const cpx<T> zk = pre(k, xk, xm), zm = pre(M - k, xm, xk);
+      r[k] = zk;
+      r[M - k] = zm;
+    }
+  }
+  __syncthreads();
+  fft_lines<T, M, N1, NTP, true>(lds, LS, N1, tw_half);
+  const int64_t off = site * g.P + (int64_t)z * PLANE;
+  const T* __restrict__ t = static_cast<const T*>(d.t) + off;
+  T* __restrict__ r = static_cast<T*>(d.r) + off;  // (BF16 slots: addressed in e below)
+  T* __restrict__ un = d.unew ? static_cast<T*>(d.unew) + off : nullptr;  // null: no u
+  const bool r16 = d.r16 != 0;
+  const int nd = d.ndot;
+  const int mode = d.mode;
+  const T lam = T(d.lam);
+  double mx = 0.0, ss = 0.0;
+  double dot[H];
+#pragma unroll
+  for (int j = 0; j < H; ++j) dot[j] = 0.0;
+  // one vector of VW points at e: t' from LDS, the streams t, F(t_b)
+  // (mode 5) and the history r_j (rhj(j), unpacked on use)
+  auto point = [&](int e, const Vec16<T>& tv, const Vec16<T>& f, auto&& rhj) {
+    Vec16<T> zv;  // t' at points e.. of row e / N0
+    {
+      const cpx<T>* zrow = lds + (e / N0) * LS;
+      const int n0 = (e % N0) / 2;
+#pragma unroll
+      for (int h = 0; h < VW / 2; ++h) {
+        const cpx<T> zz = zrow[n0 + h];
+        zv.v[2 * h] = zz.re;
+        zv.v[2 * h + 1] = zz.im;
+      }
+    }
+    Vec16<T> rv, uv;
+#pragma unroll
+    for (int k = 0; k < VW; ++k) {
+      if (mode == 5) rv.v[k] = f.v[k] + zv.v[k] - tv.v[k];  // F(t_b) + K dc - delta
+      else if (mode == 0 || mode == 6) rv.v[k] = zv.v[k] - tv.v[k];
+      else if (mode == 1)
+        rv.v[k] = T(gld(d.dt + off + e + k)) * zv.v[k] + T(gld(d.gv + off + e + k)) - tv.v[k];
+      else rv.v[k] = zv.v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < VW; ++k) uv.v[k] = tv.v[k] + lam * rv.v[k];
+    if (un) st16(un + e, uv);
+#pragma unroll
+    for (int k = 0; k < VW; ++k) mx = fmax(mx, fabs((double)rv.v[k]));
+    bool stored = false;
+    if constexpr (sizeof(T) == 4) {
+      if (r16) {  // store rounded, and form the Gram row from the rounded values
+        BF4 b4;
+        b4.lo = bf16_bits(rv.v[0]) | (bf16_bits(rv.v[1]) << 16);
+        b4.hi = bf16_bits(rv.v[2]) | (bf16_bits(rv.v[3]) << 16);
+        stbf4(static_cast<uint16_t*>(d.r) + off + e, b4);
+        rv.v[0] = bf16_val(b4.lo & 0xFFFFu);
+        rv.v[1] = bf16_val(b4.lo >> 16);
+        rv.v[2] = bf16_val(b4.hi & 0xFFFFu);
+        rv.v[3] = bf16_val(b4.hi >> 16);
+        stored = true;
+      }
+    }
+    if (!stored) st16(r + e, rv);
+    T s2 = T(0);
+#pragma unroll
+    for (int k = 0; k < VW; ++k) s2 += rv.v[k] * rv.v[k];
+    ss += (double)s2;
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+      if (j < nd) {
+        const Vec16<T> x = rhj(j);
+        T pj = T(0);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) pj += rv.v[k] * x.v[k];
+        dot[j] += (double)pj;
+      }
+  };
+  bool done = false;
+  if constexpr (sizeof(T) == 4) {
+    if (r16) {
+      // BF16 history (the FP32 phase): two vectors per step, their streams
+      // in flight together, the history kept packed (two VGPRs per vector)
+      for (int e0 = threadIdx.x * VW; e0 < PLANE; e0 += 2 * NTP * VW) {
+        Vec16<T> tv[2], f[2];
+        BF4 rb[2][H];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int e = e0 + q * NTP * VW;
+          if (!WHOLE && e >= PLANE) continue;
+          tv[q] = ld16(t + e);
+          if (mode == 5) f[q] = ld16(static_cast<const T*>(d.fb) + off + e);
+#pragma unroll
+          for (int j = 0; j < H; ++j)
+            if (j < nd) rb[q][j] = ldbf4(static_cast<const uint16_t*>(d.rh[j]) + off + e);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int e = e0 + q * NTP * VW;
+          if (!WHOLE && e >= PLANE) continue;
+          point(e, tv[q], f[q], [&](int j) {
+            Vec16<T> x;
+            x.v[0] = bf16_val(rb[q][j].lo & 0xFFFFu);
+            x.v[1] = bf16_val(rb[q][j].lo >> 16);
+            x.v[2] = bf16_val(rb[q][j].hi & 0xFFFFu);
+            x.v[3] = bf16_val(rb[q][j].hi >> 16);
+            return x;
+          });
+        }
+      }
+      done = true;
+    }
+  }
+  if (!done) {
+    for (int e = threadIdx.x * VW; e < PLANE; e += NTP * VW) {
+      const Vec16<T> tv = ld16(t + e);
+      Vec16<T> f;
+      if (mode == 5) f = ld16(static_cast<const T*>(d.fb) + off + e);
+      Vec16<T> rh[H];
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+        if (j < nd) rh[j] = ld16(static_cast<const T*>(d.rh[j]) + off + e);
+      point(e, tv, f, [&](int j) { return rh[j]; });
+    }
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  mx = warp_max(mx);
+  ss = warp_sum(ss);
+  if (lane == 0) { red[0][w] = mx; red[1][w] = ss; }
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    if (j < nd) {
+      const double s2 = warp_sum(dot[j]);
+      if (lane == 0) red[2 + j][w] = s2;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 + nd) {
+    const int q = threadIdx.x;
+    double a = red[q][0];
+    for (int i = 1; i < NTP / 64; ++i) a = (q == 0) ? fmax(a, red[q][i]) : a + red[q][i];
     gst(d.part + ((int64_t)site * gridDim.x + blockIdx.x) * NPART + q, a);
   }
 }

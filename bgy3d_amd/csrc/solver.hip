@@ -171,6 +171,42 @@ void launch_zreg_n(int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd, c
     if (split) hipLaunchKernelGGL((zstar_reg<T, N2, L, 5, true>), gn, dim3(NT), 0, s, sd, g, tw, chi, tau);
   }
 }
+// The FP32 residual on xy planes (closure_r2c_xy, zstar, c2r_xy_resid:
+// three launches, the y transforms inside the plane kernels) when the
+// planes are square with 64, 96 or 128 points per side; otherwise, and in
+// FP64 (a 128^2 plane would take 133 KB of LDS), the five-launch chain.
+// RISM3D_PLANES=0 keeps the five-launch chain in FP32 too.
+bool use_planes(const GridArgs& g, bool fp32, int on) {
+  return on && fp32 && g.special == 0 && g.N0 == g.N1 &&
+         (g.N0 == 64 || g.N0 == 96 || g.N0 == 128);
+}
+template <typename T, int N, int H>
+void launch_planes_h(bool inv, dim3 grid, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
+                     const cpx<T>* twh, const cpx<T>* twf, const cpx<T>* twy) {
+  if (inv)
+    hipLaunchKernelGGL((c2r_xy_resid<T, N, N, H>), grid, dim3(NTP), 0, s, sd, g, twh, twf, twy);
+  else
+    hipLaunchKernelGGL((closure_r2c_xy<T, N, N, H>), grid, dim3(NTP), 0, s, sd, g, twh, twf, twy);
+}
+template <typename T, int N>
+void launch_planes_n(bool inv, dim3 grid, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
+                     const cpx<T>* twh, const cpx<T>* twf, const cpx<T>* twy, int hist) {
+  if (hist <= 8) launch_planes_h<T, N, 8>(inv, grid, s, sd, g, twh, twf, twy);
+  else if (hist <= 10) launch_planes_h<T, N, 10>(inv, grid, s, sd, g, twh, twf, twy);
+  else launch_planes_h<T, N, MAX_DIIS>(inv, grid, s, sd, g, twh, twf, twy);
+}
+// Grid: x = z plane, y = site, z = solute.
+template <typename T>
+void launch_planes(bool inv, int B, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
+                   const cpx<T>* twh, const cpx<T>* twf, const cpx<T>* twy, int hist) {
+  const dim3 grid(g.N2, g.m, B);
+  switch (g.N0) {
+    case 64: launch_planes_n<T, 64>(inv, grid, s, sd, g, twh, twf, twy, hist); break;
+    case 96: launch_planes_n<T, 96>(inv, grid, s, sd, g, twh, twf, twy, hist); break;
+    case 128: launch_planes_n<T, 128>(inv, grid, s, sd, g, twh, twf, twy, hist); break;
+  }
+}
+
 // zstar_reg for the solute residual with 3 or 5 solvent sites (RISM3D_ZSTAR_LDS=1 keeps
 // the all-sites-in-LDS kernel), zstar otherwise.
 bool use_zreg(const GridArgs& g) {
@@ -540,6 +576,7 @@ struct Problem {
     }
   } graphs;
   int use_graphs = env_int("RISM3D_GRAPHS", 1);
+  int planes = env_int("RISM3D_PLANES", 1);  // FP32 chain on xy planes (use_planes)
   int noalloc = 0;  // buffers from the fixed scratch: no pinned staging, no graphs
 };
 
@@ -570,15 +607,17 @@ void download_out(Problem& pb, std::vector<double>& out) {
 
 // Per-kernel HIP-event timing of the residual chain (rism3d_profile_*).
 // Kernel ids: 0 closure_r2c_x, 1 fft_y (forward), 2 zstar, 3 fft_y
-// (inverse), 4 c2r_x_resid; slots 0-4 count the FP64 launches, 5-9 the
+// (inverse), 4 c2r_x_resid, 5 closure_r2c_xy, 6 c2r_xy_resid; slots
+// 0..PROF_KERNELS-1 count the FP64 launches, the next PROF_KERNELS the
 // FP32 ones.  Bytes are the algorithmic (compulsory) HBM traffic of each
 // launch: every array the kernel must read or write, once.
+constexpr int PROF_KERNELS = 8;
 struct Profile {
   bool on = false;
   hipEvent_t ev[5][2] = {};
-  double ms[10] = {};
-  double bytes[10] = {};
-  long launches[10] = {};
+  double ms[2 * PROF_KERNELS] = {};
+  double bytes[2 * PROF_KERNELS] = {};
+  long launches[2 * PROF_KERNELS] = {};
 };
 Profile g_prof;
 
@@ -604,8 +643,15 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
     if (pf.on) HIPCHK(hipEventRecord(pf.ev[k][which], pb.s));
   };
   const int hist = hist_of(sd);
+  const bool planes = use_planes(g, fp32, pb.planes);
+  const int nparts = planes ? g.m * g.N2 : pb.nparts;  // partial rows per solute
   auto chain = [&]() {
-    if (fp32) {
+    if (planes) {
+      using T = float;
+      mark(0, 0); launch_planes<T>(false, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, pb.tw.f[2].p, hist); mark(0, 1);
+      mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
+      mark(4, 0); launch_planes<T>(true, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, pb.tw.f[2].p, hist); mark(4, 1);
+    } else if (fp32) {
       using T = float;
       constexpr int LN = y_lines<T>();
       const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
@@ -624,7 +670,7 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
       mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(3, 1);
       mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist); mark(4, 1);
     }
-    hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, pb.d_sd.p, pb.nparts, pb.d_out.p);
+    hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, pb.d_sd.p, nparts, pb.d_out.p);
   };
   if (pb.use_graphs && !pf.on) {
     auto& G = pb.graphs;
@@ -668,13 +714,15 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
                            na * (2 * mK * 2 * sc + Kv * 2 * sc) + tables,
                            na * 2 * mK * 2 * sc, b4};
     HIPCHK(hipEventSynchronize(pf.ev[4][1]));
-    const int slot = fp32 ? 5 : 0;
+    const int slot = fp32 ? PROF_KERNELS : 0;
     for (int k = 0; k < 5; ++k) {
+      if (planes && (k == 1 || k == 3)) continue;  // the y passes are inside the plane kernels
+      const int id = planes ? (k == 0 ? 5 : k == 4 ? 6 : k) : k;
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, pf.ev[k][0], pf.ev[k][1]));
-      pf.ms[slot + k] += ms;
-      pf.bytes[slot + k] += per[k];
-      pf.launches[slot + k] += 1;
+      pf.ms[slot + id] += ms;
+      pf.bytes[slot + id] += per[k];
+      pf.launches[slot + id] += 1;
     }
   }
 }
@@ -2663,7 +2711,7 @@ int rism3d_profile_start(int device) {
   for (int k = 0; k < 5; ++k)
     for (int w = 0; w < 2; ++w)
       if (!pf.ev[k][w] && hipEventCreate(&pf.ev[k][w]) != hipSuccess) return RISM3D_NO_DEVICE;
-  for (int k = 0; k < 10; ++k) {
+  for (int k = 0; k < 2 * rism::PROF_KERNELS; ++k) {
     pf.ms[k] = 0;
     pf.bytes[k] = 0;
     pf.launches[k] = 0;
@@ -2675,7 +2723,7 @@ int rism3d_profile_start(int device) {
 int rism3d_profile_read(double* ms, double* bytes, long* launches) {
   std::lock_guard<std::mutex> lock(rism::g_mutex);
   auto& pf = rism::g_prof;
-  for (int k = 0; k < 10; ++k) {
+  for (int k = 0; k < 2 * rism::PROF_KERNELS; ++k) {
     if (ms) ms[k] = pf.ms[k];
     if (bytes) bytes[k] = pf.bytes[k];
     if (launches) launches[k] = pf.launches[k];

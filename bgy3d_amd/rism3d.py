@@ -108,7 +108,9 @@ EXPORTS = ("rism3d_options_default", "rism3d_solute_solve", "rism3d_batch_solve"
            "rism3d_profile_start",
            "rism3d_profile_read")
 
-KERNELS = ("closure_r2c_x", "fft_y_fwd", "zstar", "fft_y_inv", "c2r_x_resid")
+# rism3d_profile_read's kernel ids 0-7 (7 unused)
+KERNELS = ("closure_r2c_x", "fft_y_fwd", "zstar", "fft_y_inv", "c2r_x_resid", "closure_r2c_xy",
+           "c2r_xy_resid", "unused")
 
 _lib = None
 
@@ -546,12 +548,13 @@ def profile_read():
     """{kernel<type>: (ms, algorithmic bytes, launches)} since profile_start,
     one entry per residual kernel and working precision, e.g.
     "closure_r2c_x<float>"; instantiations never launched are left out."""
-    ms = np.zeros(10)
-    by = np.zeros(10)
-    n = (ctypes.c_long * 10)()
+    nk = len(KERNELS)
+    ms = np.zeros(2 * nk)
+    by = np.zeros(2 * nk)
+    n = (ctypes.c_long * (2 * nk))()
     load().rism3d_profile_read(_dptr(ms), _dptr(by), n)
     out = {}
-    for slot, typ in ((0, "double"), (5, "float")):
+    for slot, typ in ((0, "double"), (nk, "float")):
         for i, k in enumerate(KERNELS):
             if n[slot + i]:
                 out[f"{k}<{typ}>"] = (float(ms[slot + i]), float(by[slot + i]), int(n[slot + i]))

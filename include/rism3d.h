@@ -361,14 +361,16 @@ int rism3d_solvent_chi(const rism3d_problem *pd,
 
 /* Kernel timing (no counterpart in the reference, which has no in-code
    profiling; for benchmarks).  After rism3d_profile_start, every residual
-   evaluation brackets its five kernels with HIP events on the solver's
-   stream:  0 closure_r2c_x, 1 fft_y forward, 2 zstar (z FFTs + chi
-   contraction), 3 fft_y inverse, 4 c2r_x_resid.  rism3d_profile_read
-   returns, per kernel and working precision, the summed event time [ms],
-   the summed algorithmic HBM bytes (each array the kernel must read or
-   write, counted once) and the launch count, then stops profiling.
-   Arrays of 10: entries 0-4 the FP64 launches of kernels 0-4, entries 5-9
-   their FP32 launches. */
+   evaluation brackets its kernels with HIP events on the solver's stream
+   (launched one by one, not as a graph):  0 closure_r2c_x, 1 fft_y
+   forward, 2 zstar (z FFTs + chi contraction), 3 fft_y inverse,
+   4 c2r_x_resid, and in the three-launch FP32 chain on xy planes
+   5 closure_r2c_xy (closure + x and y transforms), 2 zstar,
+   6 c2r_xy_resid.  rism3d_profile_read returns, per kernel and working
+   precision, the summed event time [ms], the summed algorithmic HBM bytes
+   (each array the kernel must read or write, counted once) and the launch
+   count, then stops profiling.  Arrays of 16: entries 0-7 the FP64
+   launches of kernels 0-7, entries 8-15 their FP32 launches (7 unused). */
 int rism3d_profile_start(int device);
 int rism3d_profile_read(double *ms, double *bytes, long *launches);
 

@@ -101,6 +101,41 @@ def test_residual_mixed(chi_w):
     assert err < 2e-5 * max(1.0, np.max(np.abs(r_ref)))
 
 
+@pytest.mark.parametrize("N", [64, 96, 128])
+def test_residual_mixed_planes(chi_w, N):
+    """The FP32 residual on xy planes (three launches, the default for square
+    planes of 64-128 points) against the five-launch chain
+    (RISM3D_PLANES=0) and the oracle, charged 18-site solute."""
+    import os
+    print(f"case residual-mixed planes n{N}-kh charged", flush=True)
+    R = _R()
+    from bgy3d_amd import synthetic as S
+    sol = S.solute(5)
+    s = _settings(N, L=10.0, closure="KH", precision="mixed")
+    pd = _oracle_problem(s)
+    ref = O.Solute3D(pd, _sites(W), sol, chi_w)
+    t = _smooth_t(pd, 3, seed=11)
+    r_ref = ref.iterate(t)
+    old = os.environ.get("RISM3D_PLANES")
+    try:
+        os.environ["RISM3D_PLANES"] = "1"
+        r_pl = R.residual(sol, W, s, chi_w, t)
+        os.environ["RISM3D_PLANES"] = "0"
+        r_rows = R.residual(sol, W, s, chi_w, t)
+    finally:
+        if old is None:
+            os.environ.pop("RISM3D_PLANES", None)
+        else:
+            os.environ["RISM3D_PLANES"] = old
+    scale = max(1.0, np.max(np.abs(r_ref)))
+    e_pl, e_rows = np.max(np.abs(r_pl - r_ref)), np.max(np.abs(r_rows - r_ref))
+    e_pr = np.max(np.abs(r_pl - r_rows))
+    print(f"  max|F| {scale:.3e} planes-oracle {e_pl:.3e} rows-oracle {e_rows:.3e} "
+          f"planes-rows {e_pr:.3e}", flush=True)
+    assert e_pl < 2e-5 * scale and e_rows < 2e-5 * scale
+    assert e_pr < 2e-5 * scale
+
+
 SOLVE_CASES = [
     ("n32-hnc", 32, "HNC", True, W),
     ("n48-kh", 48, "KH", True, W),
