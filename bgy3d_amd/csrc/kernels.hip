@@ -604,21 +604,45 @@ constexpr int zreg_lines() {
   return L;
 }
 
-template <typename T, int N2, int LINES, int MS, bool NYQ>
-__global__ void __launch_bounds__(NT)
+template <typename T, int N2, int LINES, int MS, bool NYQ, int NTZ = NT>
+__global__ void __launch_bounds__(NTZ)
 zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ tw,
           const T* __restrict__ chi3, const T* __restrict__ tau3) {
   constexpr int PTS = N2 * LINES;
-  constexpr int PPT = (PTS + NT - 1) / NT;
+  constexpr int PPT = (PTS + NTZ - 1) / NTZ;
   __shared__ cpx<T> lds[PTS];  // interleaved: point z of line l at z * LINES + l
-  const int b = (NYQ || g.zorder) ? blockIdx.z : blockIdx.x;
+  // Block order.  zorder 2 (1D grid): consecutive blocks go to the XCDs in
+  // turn, and each XCD walks all solutes of one (ky, kx tile) before the
+  // next, so the tile's chi and tau stay in that XCD's L2 for every solute.
+  // zorder 1: x = kx tile, y = ky, z = solute; zorder 0: x = solute,
+  // y = kx tile, z = ky.
+  int b, tile, ky;
+  if (NYQ) {
+    b = blockIdx.z;
+    tile = 0;
+    ky = blockIdx.x * LINES;
+  } else if (g.zorder == 2) {
+    const int lin = blockIdx.x, xcd = lin & 7, slot = lin >> 3;
+    const int gt = (slot / g.zB) * 8 + xcd;  // global tile, ky-major
+    b = slot - (slot / g.zB) * g.zB;
+    tile = gt % g.ztiles;
+    ky = gt / g.ztiles;
+  } else if (g.zorder == 1) {
+    b = blockIdx.z;
+    tile = blockIdx.x;
+    ky = blockIdx.y;
+  } else {
+    b = blockIdx.x;
+    tile = blockIdx.y;
+    ky = blockIdx.z;
+  }
   const SoluteDesc& d = sd[b];
   if (!d.active) return;
   // NYQ: lines are the rows y0 + l of the Nyquist column kx = N0/2 (grid
   // x = ceil(N1 / LINES), y = 1, z = solute); otherwise the kx columns
   // kx0 + l of row y.
-  const int y = NYQ ? blockIdx.x * LINES : (g.zorder ? blockIdx.y : blockIdx.z);
-  const int kx0 = NYQ ? g.NX - 1 : (g.zorder ? blockIdx.x : blockIdx.y) * LINES;
+  const int y = ky;
+  const int kx0 = NYQ ? g.NX - 1 : tile * LINES;
   const int64_t zs = (int64_t)g.N1 * g.NH;  // z stride
   const int64_t ls = NYQ ? g.NH : 1;        // line stride
   const int64_t base = (int64_t)y * g.NH + kx0;
@@ -626,40 +650,43 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
   const int nl = NYQ ? min(LINES, g.N1 - y) : min(LINES, g.NX - kx0);  // valid lines
   const int tid = threadIdx.x;
   cpx<T> c[MS][PPT];
+  // every site's loads first, staged in its registers of c (its FFT result
+  // lands there after its pass), so the later sites' loads overlap the
+  // earlier sites' transforms; a partial tile reads only its valid lines
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+      const int e = tid + p * NTZ;
+      if (PPT * NTZ == PTS || e < PTS) {
+        const int z = e / LINES, l = e - z * LINES;
+        if (nl == LINES || l < nl) c[s][p] = ldc(W + s * g.K + z * zs + l * ls);
+      }
+    }
+  }
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
     cpx<T>* Ws = W + s * g.K;
     {
-      // all loads first, staged in this site's registers of c (its FFT
-      // result lands there after the pass); lines past nl are zeroed
       cpx<T>* v = c[s];
-#pragma unroll
-      for (int p = 0; p < PPT; ++p) {
-        const int e = tid + p * NT;
-        if (PPT * NT == PTS || e < PTS) {
-          const int z = e / LINES, l = e - z * LINES;
-          // a partial tile reads only its valid lines
-          if (nl == LINES || l < nl) v[p] = ldc(Ws + z * zs + l * ls);
-        }
-      }
       if (d.mode == 4) {  // inverse only: the loaded spectrum goes to the inverse pass
 #pragma unroll
         for (int p = 0; p < PPT; ++p)
-          if ((tid + p * NT) % LINES >= nl) v[p] = cpx<T>{T(0), T(0)};
+          if ((tid + p * NTZ) % LINES >= nl) v[p] = cpx<T>{T(0), T(0)};
         continue;
       }
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
-        const int e = tid + p * NT;
-        if (PPT * NT == PTS || e < PTS) lds[e] = ((e & (LINES - 1)) < nl) ? v[p] : cpx<T>{T(0), T(0)};
+        const int e = tid + p * NTZ;
+        if (PPT * NTZ == PTS || e < PTS) lds[e] = ((e & (LINES - 1)) < nl) ? v[p] : cpx<T>{T(0), T(0)};
       }
     }
     __syncthreads();
-    fft_lines_il<T, N2, LINES, NT, false>(lds, nl, tw);
+    fft_lines_il<T, N2, LINES, NTZ, false>(lds, nl, tw);
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
-      const int e = tid + p * NT;
-      if (PPT * NT == PTS || e < PTS) {
+      const int e = tid + p * NTZ;
+      if (PPT * NTZ == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
         c[s][p] = lds[e];
         if (d.mode == 3 && l < nl) stc(Ws + z * zs + l * ls, c[s][p]);  // forward spectrum only
@@ -672,8 +699,8 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
   const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + base;
 #pragma unroll
   for (int p = 0; p < PPT; ++p) {
-    const int e = tid + p * NT;
-    if (d.mode == 4 || !(PPT * NT == PTS || e < PTS)) continue;
+    const int e = tid + p * NTZ;
+    if (d.mode == 4 || !(PPT * NTZ == PTS || e < PTS)) continue;
     const int z = e / LINES, l = e - z * LINES;
     if (l >= nl) continue;
     const int64_t kk = base + z * zs + l * ls;  // k-grid index of this point
@@ -726,18 +753,18 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
     cpx<T>* Ws = W + s * g.K;
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
-      const int e = tid + p * NT;
-      if (PPT * NT == PTS || e < PTS) {
+      const int e = tid + p * NTZ;
+      if (PPT * NTZ == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
         lds[e] = c[s][p];
       }
     }
     __syncthreads();
-    fft_lines_il<T, N2, LINES, NT, true>(lds, nl, tw);
+    fft_lines_il<T, N2, LINES, NTZ, true>(lds, nl, tw);
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
-      const int e = tid + p * NT;
-      if (PPT * NT == PTS || e < PTS) {
+      const int e = tid + p * NTZ;
+      if (PPT * NTZ == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
         if (l < nl) stc(Ws + z * zs + l * ls, lds[e]);
       }

@@ -68,7 +68,8 @@ struct GridArgs {
   int renorm;          // 1: tau, 0: Ng long-range form
   int closure;         // enum rism3d_closure
   int pse_order;       // n of PSE-n, 1 for KH
-  int zorder;          // zstar grid order (see zstar)
+  int zorder;          // zstar grid order (see zstar_reg)
+  int zB, ztiles;      // zorder 2: solutes and kx tiles of the launch
   double q[MAX_SITES]; // solvent charges
   // zstar variants (0 solute residual): 1 solvent-solvent OZ on site-site
   // pairs (msol sites, pair p = i(i+1)/2 + j), 2 Poisson potential of a

@@ -131,7 +131,7 @@ void launch_zstar_l(int N2, dim3 grid, size_t lds, hipStream_t s, const SoluteDe
   }
 }
 // Tuning knobs, read once: RISM3D_ZSTAR_LINES (kx columns per zstar
-// workgroup, 2..16) and RISM3D_ZSTAR_ORDER (0 or 1, see zstar).
+// workgroup, 2..16) and RISM3D_ZSTAR_ORDER (0, 1 or 2, see zstar_reg).
 int env_int(const char* name, int dflt) {
   const char* v = std::getenv(name);
   return v ? std::atoi(v) : dflt;
@@ -147,10 +147,11 @@ int zstar_lines(int N2, int m) {
   while (L > 2 && (size_t)m * L * lstride_for(N2) * sizeof(cpx<T>) > 64 * 1024) L /= 2;
   return L;
 }
-template <typename T, int N2>
+template <typename T, int N2, int NTZ = NT>
 void launch_zreg_n(int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
                    const cpx<T>* tw, const T* chi, const T* tau) {
-  constexpr int L = zreg_lines<T, N2>();
+  // NTZ threads: the same points per thread over NTZ / NT times the columns
+  constexpr int L = zreg_lines<T, N2>() * (NTZ / NT);
   // In FP32 the Nyquist column kx = N0/2 gets a launch of its own (NYQ,
   // LINES rows per workgroup) when the tiles of LINES columns end just
   // before it, instead of a tile that holds it alone (one workgroup in five
@@ -161,14 +162,19 @@ void launch_zreg_n(int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd, c
   static const int nyq_on = env_int("RISM3D_NYQ", 1);
   const bool split = nyq_on && sizeof(T) == 4 && NX > 1 && (NX - 1) % L == 0;
   const int tiles = split ? (NX - 1) / L : (NX + L - 1) / L;
-  const dim3 grid = g.zorder ? dim3(tiles, N1, B) : dim3(B, tiles, N1);
+  GridArgs gz = g;
+  if (gz.zorder == 2 && (tiles * N1) % 8 != 0) gz.zorder = 1;  // whole tiles per XCD
+  gz.zB = B;
+  gz.ztiles = tiles;
+  const dim3 grid = gz.zorder == 2 ? dim3(tiles * N1 * B)
+                  : gz.zorder ? dim3(tiles, N1, B) : dim3(B, tiles, N1);
   const dim3 gn((N1 + L - 1) / L, 1, B);
   if (g.m == 3) {
-    hipLaunchKernelGGL((zstar_reg<T, N2, L, 3, false>), grid, dim3(NT), 0, s, sd, g, tw, chi, tau);
-    if (split) hipLaunchKernelGGL((zstar_reg<T, N2, L, 3, true>), gn, dim3(NT), 0, s, sd, g, tw, chi, tau);
+    hipLaunchKernelGGL((zstar_reg<T, N2, L, 3, false, NTZ>), grid, dim3(NTZ), 0, s, sd, gz, tw, chi, tau);
+    if (split) hipLaunchKernelGGL((zstar_reg<T, N2, L, 3, true, NTZ>), gn, dim3(NTZ), 0, s, sd, gz, tw, chi, tau);
   } else {
-    hipLaunchKernelGGL((zstar_reg<T, N2, L, 5, false>), grid, dim3(NT), 0, s, sd, g, tw, chi, tau);
-    if (split) hipLaunchKernelGGL((zstar_reg<T, N2, L, 5, true>), gn, dim3(NT), 0, s, sd, g, tw, chi, tau);
+    hipLaunchKernelGGL((zstar_reg<T, N2, L, 5, false, NTZ>), grid, dim3(NTZ), 0, s, sd, gz, tw, chi, tau);
+    if (split) hipLaunchKernelGGL((zstar_reg<T, N2, L, 5, true, NTZ>), gn, dim3(NTZ), 0, s, sd, gz, tw, chi, tau);
   }
 }
 // The FP32 residual on xy planes (closure_r2c_xy, zstar, c2r_xy_resid:
@@ -1791,7 +1797,7 @@ int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
   g.beta = pd->beta;
   g.rho = pd->rho;
   g.renorm = 0;
-  g.zorder = env_int("RISM3D_ZSTAR_ORDER", 1) ? 1 : 0;
+  g.zorder = std::min(2, std::max(0, env_int("RISM3D_ZSTAR_ORDER", 2)));
   g.closure = pd->closure;
   g.pse_order = pd->closure == RISM3D_CLOSURE_KH ? 1 : (pd->closure >= RISM3D_CLOSURE_PSE0 ? pd->closure - RISM3D_CLOSURE_PSE0 : 0);
   for (int i = 0; i < m; ++i) g.q[i] = solvent[i].charge;
@@ -1985,7 +1991,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   g.tau_scale = pd.beta * EPSILON0INV / L3;
   g.beta = pd.beta;
   g.renorm = (in.opt.renorm && !in.opt.chi3d) ? 1 : 0;
-  g.zorder = env_int("RISM3D_ZSTAR_ORDER", 1) ? 1 : 0;
+  g.zorder = std::min(2, std::max(0, env_int("RISM3D_ZSTAR_ORDER", 2)));
   g.closure = pd.closure;
   g.pse_order = pd.closure == RISM3D_CLOSURE_KH ? 1 : (pd.closure >= RISM3D_CLOSURE_PSE0 ? pd.closure - RISM3D_CLOSURE_PSE0 : 0);
   for (int i = 0; i < in.m; ++i) g.q[i] = in.solvent[i].charge;
