@@ -227,8 +227,13 @@ def make_options(settings=None, **kw):
     if s.get("no-renorm"):
         opt.renorm = 0
     if "snes-solver" in s:
-        opt.solver = {"mdiis": SOLVER_MDIIS, "picard": SOLVER_PICARD, "newton": SOLVER_NEWTON,
-                      "jager": SOLVER_JAGER, "trial": SOLVER_TRIAL}[s["snes-solver"]]
+        # bgy3d_snes_default's names (bgy3d-snes.c:16-35) plus "mdiis", the
+        # default here; an unknown name is an error, as there
+        names = {"mdiis": SOLVER_MDIIS, "picard": SOLVER_PICARD, "newton": SOLVER_NEWTON,
+                 "jager": SOLVER_JAGER, "trial": SOLVER_TRIAL}
+        if s["snes-solver"] not in names:
+            raise ValueError(f"No such SNES solver: {s['snes-solver']}")
+        opt.solver = names[s["snes-solver"]]
     for key, field in (("precision", "precision"), ("polish-iters", "polish_iters"),
                        ("mixed-tol", "mixed_tol"), ("diis-depth", "diis_depth"),
                        ("diis-step", "diis_step"), ("verbose", "verbose"),

@@ -186,3 +186,13 @@ def test_cache_release_without_work():
     """rism3d_cache_release with an empty cache (no device needed)."""
     R = _R()
     assert R.load().rism3d_cache_release() == R.OK
+
+
+def test_unknown_snes_solver_is_an_error():
+    """bgy3d_snes_default (bgy3d-snes.c:16-35) exits on an unknown solver
+    name; the options mapping refuses it before any device work."""
+    from bgy3d_amd import rism3d as R
+    for name in ("newton", "picard", "jager", "trial", "mdiis"):
+        R.make_options({"snes-solver": name})
+    with pytest.raises(ValueError, match="No such SNES solver"):
+        R.make_options({"snes-solver": "broyden"})
