@@ -8,9 +8,10 @@ adjoint gradient the same call returns (rism3d_result.dmu_dsite: the
 transposed linearised closure + RISM equation solved at the converged t,
 reusing the residual's FFT kernels, no iterates stored), so one forward
 and backward cost one solve plus one linear solve.  The gradient is exact
-for the discretised problem, for the tau-renormalised path (the default;
-`no-renorm` raises).  Tensors live on the CPU: the library owns the device
-memory.
+for the discretised problem, on the tau-renormalised path (the default)
+and the Ng path (`no-renorm`); with `precision` "mixed" the linear solve
+runs by FP32 defect correction as the forward solve does.  Tensors live on
+the CPU: the library owns the device memory.
 """
 import numpy as np
 import torch
@@ -43,8 +44,6 @@ class SoluteMu(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, params, names, solvent, settings, chi):
-        if settings.get("no-renorm"):
-            raise ValueError("the adjoint gradient is built for the tau path (no 'no-renorm')")
         want = bool(ctx.needs_input_grad[0])
         sites = sites_from_params(names, params.detach().cpu().numpy())
         out = R.hnc3d_run_solute(sites, solvent, settings, chi=chi, adjoint=want)

@@ -1758,7 +1758,8 @@ grad_field(GridArgs g, SitesDesc sd, const double* __restrict__ solvent_sse,
 // for p = x, y, z, q.
 __global__ void __launch_bounds__(NT)
 grad_tau(GridArgs g, SitesDesc sd, const cpx<double>* __restrict__ Lam,
-         const double* __restrict__ tau3, double* __restrict__ part) {
+         const double* __restrict__ tau3, const double* __restrict__ chi3,
+         double* __restrict__ part) {
   __shared__ double red[4][NT / 64];
   const int a = blockIdx.y;
   const double* s = sd.xyz_s_e_q + 6 * a;
@@ -1772,10 +1773,28 @@ grad_tau(GridArgs g, SitesDesc sd, const cpx<double>* __restrict__ Lam,
     const int iy = (y <= g.N1 / 2) ? y : y - g.N1;
     const int iz = (z <= g.N2 / 2) ? z : z - g.N2;
     const double kx = x * g.dkx, ky = iy * g.dky, kz = iz * g.dkz;
+    // the long-range weight of site i: tau_i(k) on the tau path; on the Ng
+    // path (whose residual carries -beta u_c(k) (q_i + sum_j (chi - 1)_ij q_j)
+    // with u_c = eps0^-1 4 pi e^{-k^2/4G^2}/k^2 F s, 0 at k = 0) the same
+    // factor without eps0^-1 F s, which tau_scale and the phase supply
+    double shape = 0.0;
+    if (!g.renorm) {
+      const double k2 = kmag2(kx, ky, kz);
+      shape = (k2 == 0.0) ? 0.0 : 4 * M_PI * exp(-k2 / (4 * G_COULOMB * G_COULOMB)) / k2;
+    }
     double sre = 0.0, sim = 0.0;
     for (int i = 0; i < g.m; ++i) {
       const cpx<double> L = Lam[i * g.K + e];
-      const double ti = tau3[i * g.K + e];
+      double ti;
+      if (g.renorm) {
+        ti = tau3[i * g.K + e];
+      } else {
+        double w = g.q[i];
+        for (int j = 0; j < g.m; ++j)
+          w += chi3[(int64_t)((i >= j) ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i) * g.K + e] *
+               g.q[j];
+        ti = shape * w;
+      }
       sre += ti * L.re;
       sim -= ti * L.im;
     }

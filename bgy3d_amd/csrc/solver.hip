@@ -521,6 +521,7 @@ struct SoluteState {
   int base = 0;            // mixed: rebases so far in this FP32 phase
   std::vector<int> sbase;  // mixed: base of each history slot
   double tol_lin = 0.0;    // adjoint: absolute tolerance (relative to |g|)
+  double tol_mixed = 0.0;  // mixed: FP64-checked max|F| that ends the FP32 phase
   int n_sites = 0;
   // host MDIIS state
   std::vector<int> order;     // history slots, oldest first
@@ -790,6 +791,11 @@ void rebase(Problem& pb, const Config& cf, const std::vector<int>& mask, bool ad
     d.W = s.W.p;
     d.Fs = s.fs64.p;
     d.part = s.part.p;
+    if (cf.mode == 1) {  // the adjoint's residual D K l + g - l
+      d.mode = 1;
+      d.dt = s.Dt.p;
+      d.gv = s.gmu.p;
+    }
   }
   evaluate(pb, false, sd);
   std::vector<double> out((size_t)B * NPART);
@@ -915,7 +921,13 @@ void iterate_batch(Problem& pb, const Config& cf) {
         s.phase32 = 0;
         continue;
       }
-      if (rinf[b] <= cf.mixed_tol || rinf[b] <= cf.tol) { end_fp32(b, rinf[b], r2[b]); continue; }
+      if (seed) {
+        // the adjoint starts at l = 0, where the residual is g: its
+        // tolerance is relative to max|g|
+        s.tol_mixed = cf.mode == 1 ? cf.rel_tol * rinf[b] : cf.mixed_tol;
+        if (cf.mode == 1) s.tol_lin = s.tol_mixed;
+      }
+      if (rinf[b] <= s.tol_mixed || rinf[b] <= cf.tol) { end_fp32(b, rinf[b], r2[b]); continue; }
       if (cf.precision == RISM3D_FP32 && !seed) {
         // FP32 only: no second base; the FP64 residual at the end is
         // reported, OK if it reached mixed_tol
@@ -986,9 +998,12 @@ void iterate_batch(Problem& pb, const Config& cf) {
       d.W = s.W.p;
       d.Fs = fp32 ? (const void*)s.fs32.p : (const void*)s.fs64.p;
       d.part = s.part.p;
-      d.mode = fp32 ? 5 : cf.mode;  // FP32 passes iterate the correction
+      // FP32 passes iterate the correction delta around the FP64 base:
+      // mode 5 (closure difference), or for the linear adjoint mode 1 with
+      // the base's FP64 residual as its constant term: D K delta + r_b - delta
+      d.mode = fp32 ? (cf.mode == 1 ? 1 : 5) : cf.mode;
       d.dt = s.Dt.p;
-      d.gv = s.gmu.p;
+      d.gv = (fp32 && cf.mode == 1) ? s.r64.p : s.gmu.p;
       d.fb = s.fb32.p;
       d.r16 = fp32 && pb.r16;
     }
@@ -1025,7 +1040,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
         if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; }
         else if (rinf <= (only32 ? 1e-3 : 1e-2 * s.rb_rinf)) { s.stall32++; }
         // The new entry still enters the history (the rebase moves it).
-        if ((!only32 && rinf <= 1e-3 * s.rb_rinf) || rinf <= 0.5 * cf.mixed_tol ||
+        if ((!only32 && rinf <= 1e-3 * s.rb_rinf) || rinf <= 0.5 * s.tol_mixed ||
             s.stall32 >= 12 || s.iters32 >= cf.max_iter) {
           rebase_mask[b] = 1;
           any_rebase = true;
@@ -1582,8 +1597,8 @@ void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& 
   }
   bool want_tpt = false, want_lam = false;
   // the reference's closure1 has no PY branch (closures.f90:131-153), so
-  // it has no response for PY; the adjoint is built for the tau path
-  auto wants_adj = [&](int b) { return res[b].dmu_dsite != nullptr && g.renorm; };
+  // it has no response for PY; the adjoint covers the tau and Ng paths
+  auto wants_adj = [&](int b) { return res[b].dmu_dsite != nullptr; };
   auto wants_resp = [&](int b) { return res[b].response != nullptr && g.closure != 2; };
   for (int b = 0; b < B; ++b) {
     const int n = sites[b].n;
@@ -1650,12 +1665,17 @@ void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& 
     s.since_best = 0;
     s.status = RISM3D_NONCONVERGED;
   }
+  // The linear solve l = D K l + g: in FP64, or after a mixed-precision
+  // solve by defect correction (FP32 passes on a correction around an FP64
+  // base, FP64 residual at every rebase) to the same kind of tolerance,
+  // relative to max|g|.
   Config cf = cf0;
   cf.mode = 1;
-  cf.precision = RISM3D_FP64;
+  cf.precision = cf0.precision == RISM3D_MIXED ? RISM3D_MIXED : RISM3D_FP64;
+  cf.polish = 0;
   cf.solver = RISM3D_SOLVER_MDIIS;
   cf.lam = 0.5;
-  cf.rel_tol = 1e-11;
+  cf.rel_tol = cf.precision == RISM3D_MIXED ? 1e-10 : 1e-11;
   iterate_batch(pb, cf);
   for (int b = 0; b < B; ++b) {
     auto& s = pb.st[b];
@@ -1720,7 +1740,7 @@ void gradients(Problem& pb, const Inputs& in, rism3d_result* res, const Config& 
     field_reduce(b, 1, acc);
     DevBuf<double> tpart((size_t)n * ck * 4);
     hipLaunchKernelGGL(grad_tau, dim3(ck, n), dim3(NT), 0, pb.s, g, sites[b],
-                       (const cpx<double>*)s.W.p, pb.tau3.p, tpart.p);
+                       (const cpx<double>*)s.W.p, pb.tau3.p, pb.chi3.p, tpart.p);
     std::vector<double> hp(tpart.n);
     HIPCHK(hipMemcpyAsync(hp.data(), tpart.p, hp.size() * 8, hipMemcpyDeviceToHost, pb.s));
     HIPCHK(hipMemcpyAsync(s.t64.p, tsave[b].p, mP * 8, hipMemcpyDeviceToDevice, pb.s));

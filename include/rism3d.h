@@ -180,7 +180,9 @@ typedef struct rism3d_result {
                           gradients()/energy1() hnc3d.c:1666-1742)          */
   double *dmu_dsite;   /* in: NULL or [n][6]; out: exact d mu / d(x, y, z,
                           q, sigma, epsilon) of every solute site by the
-                          adjoint of the residual (renorm path only; no
+                          adjoint of the residual, on the tau and the Ng
+                          path; its linear solve runs in FP64, or by FP32
+                          defect correction after a MIXED solve (no
                           counterpart in the reference)                     */
   int adjoint_iterations;  /* linear MDIIS steps of the adjoint solve      */
   double *response;    /* in: NULL or [n][3]; out: the reference's linear-

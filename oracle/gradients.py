@@ -20,11 +20,14 @@ TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``.
   one-solve adjoint form sum_i <A_i | dv_i> for charged ones.
 
 * ``adjoint_gradient`` is exact d mu / d(x, y, z, q, sigma, epsilon) of
-  every solute site on the renormalised path, by the adjoint of the
-  residual: with J = dF/dt = K D_t - 1 (K the chi convolution, symmetric
-  because chi(|k|) is real and even) the multiplier solves
+  every solute site on the renormalised and the Ng path, by the adjoint of
+  the residual: with J = dF/dt = K D_t - 1 (K the chi convolution,
+  symmetric because chi(|k|) is real and even) the multiplier solves
   lambda = D_t K lambda + dmu/dt, and
-  dmu/dtheta = dmu/dtheta|_t + lambda . dF/dtheta|_t.
+  dmu/dtheta = dmu/dtheta|_t + lambda . dF/dtheta|_t.  The long-range term
+  of F is -(beta eps0^-1 / L^3) w_i(k) F(k) s(k) on both paths: w_i = tau_i
+  (renorm) or 4 pi e^{-k^2/4G^2}/k^2 (q_i + sum_j (chi - 1)_ij q_j), 0 at
+  k = 0 (Ng, hnc3d.c:1226-1227, 1258-1262).
   There is no reference counterpart (the reference has position
   derivatives only, as TPT or by linear response); pinned by central
   finite differences of converged mu in tests/test_oracle_gradients.py.
@@ -208,12 +211,10 @@ def _kweights(pd):
 
 
 def adjoint_gradient(sol, t, lam_rel_tol=1e-12):
-    """Exact dmu/dtheta for a converged Solute3D ``sol`` (renorm path) at t.
+    """Exact dmu/dtheta for a converged Solute3D ``sol`` (either path) at t.
     Returns (grad[n][6] over x, y, z, q, sigma, epsilon; lambda)."""
     pd = sol.pd
     beta, rho = pd.beta, pd.rho
-    if not sol.renorm:
-        raise ValueError("adjoint gradient implemented for the renormalised path")
     meth = pd.closure
     fmu = rho * pd.h3 / beta
     bv = beta * sol.v
@@ -248,7 +249,14 @@ def adjoint_gradient(sol, t, lam_rel_tol=1e-12):
     # tau term: -tau_scale Re sum_k w tau_i s dF/dtheta conj(Lambda_i)
     Lam = O.rfft(pd, lam)
     s = O.phase_sign(pd)
-    tau0 = sol.tau_k
+    if sol.renorm:
+        tau0 = sol.tau_k
+    else:
+        k = O.kmag(pd)
+        with np.errstate(divide="ignore"):
+            u0 = 4 * math.pi * np.exp(-k * k / (4 * G * G)) / (k * k)
+        u0 = np.where(k == 0.0, 0.0, u0)
+        tau0 = u0[None] * (sol.q[:, None, None, None] + np.einsum("ij...,j->i...", sol.chi, sol.q))
     S = np.sum(tau0 * np.conj(Lam), axis=0) * s * _kweights(pd)
     bscale = beta * EPS0INV / pd.L3
     kx, ky, kz = O.k_axes(pd)

@@ -90,3 +90,45 @@ def test_adjoint_128_hnc_30_sites(chi160):
               - R.hnc3d_run_solute(moved(-1), W, s, chi=chi160)["free-energy"]) / (2 * h)
         print(f"  site {a} param {p}: adjoint {g[a, p]:.8f} fd {fd:.8f}", flush=True)
         assert abs(g[a, p] - fd) <= 1e-4 * max(abs(fd), 1e-2)
+
+
+def test_adjoint_128_hnc_30_sites_mixed(chi160):
+    """The mixed-precision leg of the same workload: mixed forward solve
+    and the adjoint by FP32 defect correction, against central differences
+    of the FP64 mu (h = 1e-4) to 1e-4 relative; forward + backward timed
+    against forward alone."""
+    print("case adjoint 128^3 HNC 30 sites mixed", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd import synthetic as S
+    sol = list(S.solute(3, n=30))
+    s64 = dict(BASE, N=128, L=20.0, closure="HNC", **{"norm-tol": 1e-11})
+    sm = dict(s64, precision="mixed", **{"norm-tol": 1e-10})
+    R.hnc3d_run_solute(sol, W, sm, chi=chi160)  # warm
+    t0 = time.time()
+    fwd = R.hnc3d_run_solute(sol, W, sm, chi=chi160)
+    t_fwd = time.time() - t0
+    t0 = time.time()
+    out = R.hnc3d_run_solute(sol, W, sm, chi=chi160, adjoint=True)
+    t_both = time.time() - t0
+    assert fwd["status"] == "OK" and out["status"] == "OK"
+    g = out["free-energy-dsite"]
+    assert np.all(np.isfinite(g))
+    print(f"  mixed: forward {t_fwd:.2f} s, forward + adjoint {t_both:.2f} s "
+          f"(adjoint iters {out['adjoint-iterations']})", flush=True)
+    h = 1e-4
+    for a, p in ((0, 0), (7, 2), (15, 3), (22, 4), (29, 5)):
+        def moved(sign):
+            x = [[n, list(xyz), sg, ep, q] for n, xyz, sg, ep, q in sol]
+            if p < 3:
+                x[a][1][p] += sign * h
+            elif p == 3:
+                x[a][4] += sign * h
+            elif p == 4:
+                x[a][2] += sign * h
+            else:
+                x[a][3] += sign * h
+            return [(n, tuple(xyz), sg, ep, q) for n, xyz, sg, ep, q in x]
+        fd = (R.hnc3d_run_solute(moved(+1), W, s64, chi=chi160)["free-energy"]
+              - R.hnc3d_run_solute(moved(-1), W, s64, chi=chi160)["free-energy"]) / (2 * h)
+        print(f"  site {a} param {p}: mixed adjoint {g[a, p]:.8f} fd {fd:.8f}", flush=True)
+        assert abs(g[a, p] - fd) <= 1e-4 * max(abs(fd), 1e-2)

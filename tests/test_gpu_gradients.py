@@ -62,6 +62,47 @@ def test_gradients_vs_oracle(closure, chi_w):
     assert e_adj < 1e-6
 
 
+@pytest.mark.parametrize("closure", ["HNC", "KH"])
+def test_adjoint_ng_path_vs_oracle(closure, chi_w):
+    """The adjoint on the Ng path (--no-renorm): its long-range weight
+    (q_i + sum_j (chi - 1)_ij q_j) 4 pi e^{-k^2/4G^2}/k^2 against the
+    oracle's, itself checked against finite differences on the CPU."""
+    print(f"case adjoint ng {closure}", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd.solutes import find_molecule
+    sol = _solute()
+    s = dict(_settings(closure), **{"no-renorm": True})
+    out = R.hnc3d_run_solute(sol, W, s, chi=chi_w, adjoint=True)
+    pd = O.Problem.from_cli(8.0, 32, README["beta"], README["rho"], closure=C.NAMES[closure],
+                            rmax=40.0, nrad=1536)
+    ref = O.Solute3D(pd, find_molecule(W), sol, chi_w, renorm=False)
+    t, _ = ref.solve(atol=1e-12)
+    adj, _ = Gr.adjoint_gradient(ref, t)
+    d_gpu = out["free-energy-dsite"]
+    scale = np.array([np.max(np.abs(adj[:, :3]))] * 3 + [np.max(np.abs(adj[:, p])) for p in (3, 4, 5)])
+    e_adj = np.max(np.abs(d_gpu - adj) / scale[None, :])
+    print(f"  adjoint rel {e_adj:.2e}  adj iters {out['adjoint-iterations']}", flush=True)
+    assert e_adj < 1e-6
+
+
+@pytest.mark.parametrize("renorm", [True, False])
+def test_mixed_adjoint_vs_fp64(renorm, chi_w):
+    """After a MIXED solve the adjoint's linear solve runs by FP32 defect
+    correction: the same gradient as the FP64 path to the accuracy of the
+    mixed forward solution (1e-4 relative per parameter kind)."""
+    print(f"case mixed adjoint renorm={renorm}", flush=True)
+    from bgy3d_amd import rism3d as R
+    sol = _solute()
+    s = dict(_settings("KH"), **{"no-renorm": not renorm})
+    a = R.hnc3d_run_solute(sol, W, s, chi=chi_w, adjoint=True)["free-energy-dsite"]
+    out = R.hnc3d_run_solute(sol, W, dict(s, precision="mixed"), chi=chi_w, adjoint=True)
+    b = out["free-energy-dsite"]
+    scale = np.array([np.max(np.abs(a[:, :3]))] * 3 + [np.max(np.abs(a[:, p])) for p in (3, 4, 5)])
+    e = np.max(np.abs(a - b) / scale[None, :])
+    print(f"  mixed vs fp64 rel {e:.2e}  adj iters {out['adjoint-iterations']}", flush=True)
+    assert e < 1e-4
+
+
 def test_adjoint_vs_finite_difference(chi_w):
     print("case adjoint-fd", flush=True)
     from bgy3d_amd import rism3d as R

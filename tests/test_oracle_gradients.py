@@ -6,6 +6,7 @@ linear-response gradient (hnc3d.c:1366-1421, 1969-2134) against both: equal
 to the exact gradient where the solute is uncharged, equal to the one-solve
 adjoint form for a charged one."""
 import numpy as np
+import pytest
 
 from conftest import CHI_W, load_chi
 from oracle import gradients as Gr
@@ -25,13 +26,16 @@ def test_capped_derivatives_fd():
         np.testing.assert_allclose(f1(r), fd, rtol=1e-5, atol=1e-4)
 
 
-def test_adjoint_matches_fd():
+@pytest.mark.parametrize("renorm", [True, False])
+def test_adjoint_matches_fd(renorm):
+    """Both paths: tau renormalisation and the Ng form (--no-renorm), whose
+    long-range weight the adjoint builds from chi and u_c(k)."""
     chi = load_chi(CHI_W)
     sol = list(S.solute(5, n=4))
     pd = O.Problem.from_cli(8.0, 32, 1.6889, 0.0333295, rmax=40.0, nrad=1536)
 
     def mu(sites):
-        s = O.Solute3D(pd, W, sites, chi)
+        s = O.Solute3D(pd, W, sites, chi, renorm=renorm)
         t, _ = s.solve(atol=1e-12)
         return s.observables(t)["mu"], s, t
 
