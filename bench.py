@@ -132,10 +132,11 @@ def cpu_baseline(solute, L_full, chi, fp64_iters, budget_s=15.0):
 
 def natural_grids(solutes, chi, args):
     """The same solutes on their own grids (synthetic.box_for: the smallest
-    of 64^3 / 96^3 / 128^3 at <= 0.35 A spacing with 8 A padding), one
-    batched launch sequence per grid size in a box that pads every member:
-    solves/s over the whole set, mixed and FP64, and |dmu| between them.
-    One untimed and one timed pass of each precision."""
+    of 64^3 / 96^3 / 128^3 at <= 0.35 A spacing with 8 A padding), in a box
+    per grid size that pads every member, all in one
+    rism3d_batch_solve_grids call (each size class a batch on its own
+    stream): solves/s over the whole set, mixed and FP64, and |dmu| between
+    them.  One untimed and one timed pass of each precision."""
     from bgy3d_amd import rism3d as R
     from bgy3d_amd import synthetic as S
     groups = {}
@@ -148,13 +149,13 @@ def natural_grids(solutes, chi, args):
         plan.append((N, groups[N], members, box))
 
     def run(precision):
-        out = [None] * len(solutes)
+        # one call: rism3d_batch_solve_grids solves the size classes together
+        st = [None] * len(solutes)
         for N, idx, members, box in plan:
-            st = settings_for(box, precision, args.diis_depth, args.diis_step, args.polish,
-                              args.mixed_tol, N=N, coarse_tol=args.coarse_tol)
-            for i, r in zip(idx, R.batch_solve(members, W, st, chi)):
-                out[i] = r
-        return out
+            for i in idx:
+                st[i] = settings_for(box, precision, args.diis_depth, args.diis_step, args.polish,
+                                     args.mixed_tol, N=N, coarse_tol=args.coarse_tol)
+        return R.batch_solve_grids(solutes, W, st, chi)
 
     run("mixed")
     run("fp64")

@@ -16,6 +16,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rism3d.h"
@@ -2487,9 +2488,12 @@ void rism3d_options_default(rism3d_options* opt) {
 int rism3d_solvent_chi_internal(const rism3d_problem* pd, int m, const rism3d_site* solvent,
                                 double dielectric, double* chi_out);
 
+// locked: take the library lock and trim the cache at the end (a lone
+// call); otherwise the caller holds the lock (rism3d_batch_solve_grids).
 static int solve_impl(const rism3d_problem* pd, int m, const rism3d_site* solvent, int B,
                       const int* n_sites, const rism3d_site* const* solutes, const double* chi_k,
-                      const rism3d_options* opt, rism3d_result* res, rism3d_restart** restart) {
+                      const rism3d_options* opt, rism3d_result* res, rism3d_restart** restart,
+                      bool locked = true) {
   if (!pd || !res || !n_sites || !solutes) return RISM3D_BAD_ARG;
   rism::Inputs in;
   in.pd = pd;
@@ -2542,7 +2546,8 @@ static int solve_impl(const rism3d_problem* pd, int m, const rism3d_site* solven
     *restart = nullptr;
   }
   try {
-    int st = rism::run(in, res, ti, restart ? &t_final : nullptr);
+    int st = locked ? rism::run(in, res, ti, restart ? &t_final : nullptr)
+                    : rism::run_impl(in, res, ti, restart ? &t_final : nullptr, nullptr);
     if (restart && !t_final.empty()) {
       rism3d_restart* r = new rism3d_restart;
       for (int d = 0; d < 3; ++d) r->N[d] = pd->N[d];
@@ -2570,6 +2575,65 @@ int rism3d_batch_solve(const rism3d_problem* pd, int m, const rism3d_site* solve
                        const int* n_sites, const rism3d_site* const* solutes, const double* chi_k,
                        const rism3d_options* opt, rism3d_result* res) {
   return solve_impl(pd, m, solvent, n_solutes, n_sites, solutes, chi_k, opt, res, nullptr);
+}
+
+int rism3d_batch_solve_grids(const rism3d_problem* pds, int m, const rism3d_site* solvent,
+                             int n_solutes, const int* n_sites,
+                             const rism3d_site* const* solutes, const double* chi_k,
+                             const rism3d_options* opt, rism3d_result* res) {
+  if (!pds || !res || !n_sites || !solutes || n_solutes < 1) return RISM3D_BAD_ARG;
+  // one chi for all: the same radial grid
+  for (int b = 1; b < n_solutes; ++b)
+    if (chi_k && (pds[b].nrad != pds[0].nrad || pds[b].rmax != pds[0].rmax)) {
+      for (int q = 0; q < n_solutes; ++q) res[q].status = RISM3D_BAD_ARG;
+      return RISM3D_BAD_ARG;
+    }
+  // size classes: solutes whose problems are identical share one batch
+  std::vector<std::vector<int>> groups;
+  for (int b = 0; b < n_solutes; ++b) {
+    bool placed = false;
+    for (auto& gr : groups)
+      if (std::memcmp(&pds[gr[0]], &pds[b], sizeof(rism3d_problem)) == 0) {
+        gr.push_back(b);
+        placed = true;
+        break;
+      }
+    if (!placed) groups.push_back({b});
+  }
+  const int G = (int)groups.size();
+  std::vector<std::vector<int>> ns(G);
+  std::vector<std::vector<const rism3d_site*>> sol(G);
+  std::vector<std::vector<rism3d_result>> rg(G);
+  std::vector<int> st(G, RISM3D_OK);
+  for (int k = 0; k < G; ++k)
+    for (int b : groups[k]) {
+      ns[k].push_back(n_sites[b]);
+      sol[k].push_back(solutes[b]);
+      rg[k].push_back(res[b]);
+    }
+  auto solve = [&](int k) {
+    st[k] = solve_impl(&pds[groups[k][0]], m, solvent, (int)groups[k].size(), ns[k].data(),
+                       sol[k].data(), chi_k, opt, rg[k].data(), nullptr, false);
+  };
+  {
+    std::lock_guard<std::mutex> lock(rism::g_mutex);
+    if (rism::g_prof.on || G == 1) {  // kernel timing records one stream at a time
+      for (int k = 0; k < G; ++k) solve(k);
+    } else {
+      // every size class on its own stream, from its own host thread: the
+      // classes' kernels share the device instead of running in turn
+      std::vector<std::thread> th;
+      for (int k = 0; k < G; ++k) th.emplace_back(solve, k);
+      for (auto& t : th) t.join();
+    }
+    rism::cache_end_call();
+  }
+  int worst = RISM3D_OK;
+  for (int k = 0; k < G; ++k) {
+    for (size_t i = 0; i < groups[k].size(); ++i) res[groups[k][i]] = rg[k][i];
+    if (st[k] != RISM3D_OK && (worst == RISM3D_OK || worst == RISM3D_NONCONVERGED)) worst = st[k];
+  }
+  return worst;
 }
 
 void rism3d_restart_destroy(rism3d_restart* restart) { delete restart; }

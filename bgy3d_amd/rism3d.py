@@ -134,6 +134,10 @@ def load():
                                            P(ctypes.c_int), P(P(Site)), dp, P(Options),
                                            P(Result)]
         lib.rism3d_batch_solve.restype = ctypes.c_int
+        lib.rism3d_batch_solve_grids.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_int,
+                                                 P(ctypes.c_int), P(P(Site)), dp, P(Options),
+                                                 P(Result)]
+        lib.rism3d_batch_solve_grids.restype = ctypes.c_int
         lib.rism3d_residual.argtypes = [P(Problem), ctypes.c_int, P(Site), ctypes.c_int,
                                         P(Site), dp, P(Options), dp, dp]
         lib.rism3d_residual.restype = ctypes.c_int
@@ -451,6 +455,47 @@ def batch_solve(solutes, solvent, settings, chi, want_g=False, g_device=None):
                                 ctypes.byref(opt), res)
     if st not in (OK, NONCONVERGED, NAN):
         raise RismError(st, "rism3d_batch_solve")  # whole-call failures only
+    out = []
+    for b in range(B):
+        d = {"free-energy": res[b].mu, "partial-molar-volume-factor": res[b].pmv_factor,
+             "iterations": res[b].iterations, "iterations-fp32": res[b].iterations_fp32,
+             "residual": res[b].residual, "status": STATUS_NAMES[res[b].status]}
+        if want_g:
+            d["GUV"] = gs[b]
+        out.append(d)
+    return out
+
+
+def batch_solve_grids(solutes, solvent, settings, chi, want_g=False):
+    """Solutes on uneven grids in one call: rism3d_batch_solve_grids.
+    settings: one settings dict per solute (its own N, L, ...; options such
+    as precision are taken from the first).  Solutes with equal problems
+    form a size class; the classes run concurrently on their own streams."""
+    lib = load()
+    if len(settings) != len(solutes):
+        raise ValueError("one settings dict per solute")
+    B = len(solutes)
+    pds = (Problem * B)()
+    for b, st in enumerate(settings):
+        pds[b] = problem_data(st)
+    opt = make_options(settings[0])
+    sv = make_sites(solvent)
+    m = len(sv)
+    arrs = [make_sites(s) for s in solutes]
+    ptrs = (ctypes.POINTER(Site) * B)(*[ctypes.cast(a, ctypes.POINTER(Site)) for a in arrs])
+    ns = (ctypes.c_int * B)(*[len(a) for a in arrs])
+    res = (Result * B)()
+    gs = []
+    if want_g:
+        for b in range(B):
+            N = pds[b].N
+            g = np.empty((m, N[2], N[1], N[0]))
+            gs.append(g)
+            res[b].g = _dptr(g)
+    chi = np.ascontiguousarray(chi, dtype=np.float64)
+    st = lib.rism3d_batch_solve_grids(pds, m, sv, B, ns, ptrs, _dptr(chi), ctypes.byref(opt), res)
+    if st not in (OK, NONCONVERGED, NAN):
+        raise RismError(st, "rism3d_batch_solve_grids")
     out = []
     for b in range(B):
         d = {"free-energy": res[b].mu, "partial-molar-volume-factor": res[b].pmv_factor,

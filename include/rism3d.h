@@ -263,6 +263,21 @@ int rism3d_batch_solve(const rism3d_problem *pd,
                        const rism3d_options *opt,
                        rism3d_result *res);
 
+/* Solutes on uneven grids in one call (no counterpart in the reference):
+   pds[b] is solute b's problem (grid, box, closure, tolerances; the same
+   radial grid rmax / nrad for all when chi_k is given).  Solutes with
+   identical problems form a size class solved as one batch
+   (rism3d_batch_solve); the classes run at the same time, each on its own
+   stream (one after another while kernel timing is on).  res[b] receives
+   each result; returns OK, or the first failure of a class. */
+int rism3d_batch_solve_grids(const rism3d_problem *pds,
+                             int m, const rism3d_site *solvent,
+                             int n_solutes, const int *n_sites,
+                             const rism3d_site *const *solutes,
+                             const double *chi_k,
+                             const rism3d_options *opt,
+                             rism3d_result *res);
+
 /* One residual evaluation F(t) = (chi - 1) * c(t) - t, replacing
    iterate_t1 (hnc3d.c:1184-1285) as called by the reference's SNES.
    t, r: double[m][N2][N1][N0].  Uses the FP32 kernels when

@@ -196,3 +196,16 @@ def test_unknown_snes_solver_is_an_error():
         R.make_options({"snes-solver": name})
     with pytest.raises(ValueError, match="No such SNES solver"):
         R.make_options({"snes-solver": "broyden"})
+
+
+def test_grids_reject_mixed_radial_grids():
+    """rism3d_batch_solve_grids shares one chi: problems on different radial
+    grids are refused before any device work."""
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd import synthetic as S
+    chi = np.zeros((3, 3, 1536))
+    a = {"N": 32, "L": 8.0, "rmax": 40.0, "nrad": 1536}
+    b = dict(a, rmax=39.0)
+    with pytest.raises(R.RismError) as e:
+        R.batch_solve_grids([S.solute(0), S.solute(1)], "water, PR-SPC/E", [a, b], chi)
+    assert e.value.status == R.BAD_ARG
