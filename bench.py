@@ -39,7 +39,8 @@ def settings_for(L_full, precision, depth=8, step=0.5, polish=3, mixed_tol=1e-5,
                  coarse_tol=0.0):
     s = {"N": N, "L": L_full / 2, "rho": 0.0333295, "beta": 1.6889,
          "rmax": 160.0, "nrad": 4096, "closure": "KH", "max-iter": 3000,
-         "norm-tol": 1e-10, "diis-depth": depth, "diis-step": step, "coarse-tol": coarse_tol}
+         "norm-tol": 1e-10, "diis-depth": depth, "diis-step": step, "coarse-tol": coarse_tol,
+         "tune": 1}
     if precision == "mixed":
         s.update({"precision": "mixed", "mixed-tol": mixed_tol, "polish-iters": polish})
     return s
@@ -208,6 +209,10 @@ def main():
     # The library keeps every buffer it releases (up to 192 GB) instead of
     # only the last call's: the bench alternates precisions and grid sizes.
     os.environ.setdefault("RISM3D_CACHE_MB", "196608")
+    # the autotuner's choices (options.tune), per grid, batch and precision
+    import tempfile
+    tune_file = os.path.join(tempfile.mkdtemp(prefix="rism3d_tune_"), "choices.txt")
+    os.environ["RISM3D_TUNE_FILE"] = tune_file
     # g of every solute is written into device buffers held by torch; torch
     # loads the HIP runtime before the library, which then uses the same one
     import torch
@@ -327,6 +332,13 @@ def main():
     })
     if not args.skip_natural:
         line["natural_grids"] = natural_grids(solutes, chi, args)
+    try:
+        with open(tune_file) as f:  # key: planes zorder zwide nyq, trial ms
+            line["tuner"] = {k: {"planes": int(a), "zorder": int(b), "zwide": int(c),
+                                 "nyq": int(d), "ms": float(e)}
+                             for k, a, b, c, d, e in (ln.split() for ln in f if ln.strip())}
+    except OSError:
+        line["tuner"] = None
     if not args.no_cpu_baseline:
         # the reference's algorithm starts from t = 0: scale the oracle's
         # per-residual cost by the residual count of a cold FP64 solve of the

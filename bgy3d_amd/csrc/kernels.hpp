@@ -70,6 +70,8 @@ struct GridArgs {
   int pse_order;       // n of PSE-n, 1 for KH
   int zorder;          // zstar grid order (see zstar_reg)
   int zB, ztiles;      // zorder 2: solutes and kx tiles of the launch
+  int nyq;             // FP32 z pass: the Nyquist column in a launch of its own
+  int zwide;           // FP32 z pass at N2 <= 128: 32 kx columns, 512 threads
   double q[MAX_SITES]; // solvent charges
   // zstar variants (0 solute residual): 1 solvent-solvent OZ on site-site
   // pairs (msol sites, pair p = i(i+1)/2 + j), 2 Poisson potential of a

@@ -159,9 +159,8 @@ void launch_zreg_n(int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd, c
   // at N0 = 128): 1.19 -> 1.08 ms per pass at 64 x 128^3 (2, 4, 8 or 16
   // rows per Nyquist workgroup alike).  In FP64, and in the y passes, the
   // separate launch cost more than the tile it replaced.  RISM3D_NYQ=0
-  // keeps it in the last tile.
-  static const int nyq_on = env_int("RISM3D_NYQ", 1);
-  const bool split = nyq_on && sizeof(T) == 4 && NX > 1 && (NX - 1) % L == 0;
+  // keeps it in the last tile (g.nyq).
+  const bool split = g.nyq && sizeof(T) == 4 && NX > 1 && (NX - 1) % L == 0;
   const int tiles = split ? (NX - 1) / L : (NX + L - 1) / L;
   GridArgs gz = g;
   if (gz.zorder == 2 && (tiles * N1) % 8 != 0) gz.zorder = 1;  // whole tiles per XCD
@@ -224,6 +223,16 @@ template <typename T>
 void launch_zstar(int N2, int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd,
                   const GridArgs& g, const cpx<T>* tw, const T* chi, const T* tau) {
   if (use_zreg(g)) {
+    if (g.zwide && sizeof(T) == 4 && N2 <= 128) {  // 512 threads over 32 kx columns
+      switch (N2) {
+        case 32: launch_zreg_n<T, 32, 2 * NT>(NX, N1, B, s, sd, g, tw, chi, tau); break;
+        case 48: launch_zreg_n<T, 48, 2 * NT>(NX, N1, B, s, sd, g, tw, chi, tau); break;
+        case 64: launch_zreg_n<T, 64, 2 * NT>(NX, N1, B, s, sd, g, tw, chi, tau); break;
+        case 96: launch_zreg_n<T, 96, 2 * NT>(NX, N1, B, s, sd, g, tw, chi, tau); break;
+        case 128: launch_zreg_n<T, 128, 2 * NT>(NX, N1, B, s, sd, g, tw, chi, tau); break;
+      }
+      return;
+    }
     switch (N2) {
 #define X(n) case n: launch_zreg_n<T, n>(NX, N1, B, s, sd, g, tw, chi, tau); break;
       RISM_FOR_N(X)
@@ -584,7 +593,17 @@ struct Problem {
     }
   } graphs;
   int use_graphs = env_int("RISM3D_GRAPHS", 1);
-  int planes = env_int("RISM3D_PLANES", 1);  // FP32 chain on xy planes (use_planes)
+  // launch configuration of the residual chain per working precision
+  // ([0] FP64, [1] FP32): defaults from the environment, or the autotuner's
+  // choice (tune)
+  struct Knobs {
+    int planes;  // FP32 chain on xy planes (use_planes)
+    int zorder;  // z-pass block order (zstar_reg)
+    int zwide;   // z-pass tile of 32 kx columns over 512 threads (FP32)
+    int nyq;     // Nyquist column in a launch of its own (FP32)
+  } knob[2] = {{0, env_int("RISM3D_ZSTAR_ORDER", 2), 0, 0},
+               {env_int("RISM3D_PLANES", 1), env_int("RISM3D_ZSTAR_ORDER", 2),
+                env_int("RISM3D_ZSTAR_WIDE", 0), env_int("RISM3D_NYQ", 1)}};
   int noalloc = 0;  // buffers from the fixed scratch: no pinned staging, no graphs
 };
 
@@ -643,7 +662,11 @@ int hist_of(const std::vector<SoluteDesc>& sd) {
 }
 
 void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
-  const GridArgs& g = pb.g;
+  GridArgs g = pb.g;
+  const Problem::Knobs& kn = pb.knob[fp32 ? 1 : 0];
+  g.zorder = std::min(2, std::max(0, kn.zorder));
+  g.zwide = kn.zwide;
+  g.nyq = kn.nyq;
   upload_sd(pb, sd);
   const dim3 gx(resid_grid_x(g), g.m, pb.B);
   Profile& pf = g_prof;
@@ -651,7 +674,7 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
     if (pf.on) HIPCHK(hipEventRecord(pf.ev[k][which], pb.s));
   };
   const int hist = hist_of(sd);
-  const bool planes = use_planes(g, fp32, pb.planes);
+  const bool planes = use_planes(g, fp32, kn.planes);
   const int nparts = planes ? g.m * g.N2 : pb.nparts;  // partial rows per solute
   auto chain = [&]() {
     if (planes) {
@@ -733,6 +756,128 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
       pf.launches[slot + id] += 1;
     }
   }
+}
+
+// ------------------------------------------------------------ autotuner
+// Candidate launch configurations of the residual chain, timed on the
+// batch's own buffers: three evaluations of F at t = 0 for every solute
+// after one untimed, launched one by one, HIP events on the solver's
+// stream.  The fastest is kept in pb.knob for this working precision.
+// Choices are cached per process by grid, sites, batch size and
+// precision, loaded from and appended to RISM3D_TUNE_FILE if it is set.
+struct TuneTable {
+  std::mutex mu;
+  std::map<std::string, std::pair<Problem::Knobs, double>> best;
+  bool loaded = false;
+};
+TuneTable& tune_table() {
+  static TuneTable t;
+  return t;
+}
+std::string tune_key(const GridArgs& g, int B, bool fp32) {
+  char buf[96];
+  std::snprintf(buf, sizeof buf, "%dx%dx%d,m%d,B%d,%s", g.N0, g.N1, g.N2, g.m, B,
+                fp32 ? "fp32" : "fp64");
+  return buf;
+}
+void tune(Problem& pb, bool fp32, int verbose) {
+  const GridArgs& g0 = pb.g;
+  const std::string key = tune_key(g0, pb.B, fp32);
+  TuneTable& T = tune_table();
+  const char* file = std::getenv("RISM3D_TUNE_FILE");
+  {
+    std::lock_guard<std::mutex> lk(T.mu);
+    if (!T.loaded && file) {  // lines: key planes zorder zwide nyq ms
+      if (FILE* f = std::fopen(file, "r")) {
+        char k[96];
+        Problem::Knobs c;
+        double ms;
+        while (std::fscanf(f, "%95s %d %d %d %d %lf", k, &c.planes, &c.zorder, &c.zwide, &c.nyq,
+                           &ms) == 6)
+          T.best[k] = {c, ms};
+        std::fclose(f);
+      }
+    }
+    T.loaded = true;
+    auto it = T.best.find(key);
+    if (it != T.best.end()) {
+      pb.knob[fp32 ? 1 : 0] = it->second.first;
+      return;
+    }
+  }
+  std::vector<Problem::Knobs> cand;
+  const bool can_planes = fp32 && use_planes(g0, true, 1);
+  const bool can_wide = fp32 && g0.N2 <= 128 && use_zreg(g0);
+  for (int planes : {1, 0})
+    for (int zorder : {2, 1})
+      for (int zwide : {0, 1})
+        for (int nyq : {1, 0}) {
+          if ((planes && !can_planes) || (zwide && !can_wide) || (!fp32 && (nyq || zwide)))
+            continue;
+          cand.push_back({planes, zorder, zwide, nyq});
+        }
+  // F(t) at t = 0 for every solute, into history slot 0
+  std::vector<SoluteDesc> sd(pb.B);
+  for (int b = 0; b < pb.B; ++b) {
+    auto& s = pb.st[b];
+    SoluteDesc& d = sd[b];
+    std::memset(&d, 0, sizeof d);
+    d.active = 1;
+    if (fp32) HIPCHK(hipMemsetAsync(s.t32.p, 0, pb.mP * 4, pb.s));
+    d.t = fp32 ? (void*)s.t32.p : (void*)s.t64.p;
+    d.r = s.R.p;
+    d.unew = s.U.p;
+    d.lam = 0.5;
+    d.bv = fp32 ? (const void*)s.bv32.p : (const void*)s.bv64.p;
+    d.W = s.W.p;
+    d.Fs = fp32 ? (const void*)s.fs32.p : (const void*)s.fs64.p;
+    d.part = s.part.p;
+  }
+  struct Pause {  // eager launches, and no kernel timing of the trials
+    Problem& pb;
+    int graphs;
+    bool prof;
+    explicit Pause(Problem& p) : pb(p), graphs(p.use_graphs), prof(g_prof.on) {
+      pb.use_graphs = 0;
+      g_prof.on = false;
+    }
+    ~Pause() {
+      pb.use_graphs = graphs;
+      g_prof.on = prof;
+    }
+  } pause(pb);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  Problem::Knobs best = pb.knob[fp32 ? 1 : 0];
+  double best_ms = INFINITY;
+  for (const auto& c : cand) {
+    pb.knob[fp32 ? 1 : 0] = c;
+    evaluate(pb, fp32, sd);  // untimed: first launch of these kernels
+    HIPCHK(hipEventRecord(e0, pb.s));
+    for (int r = 0; r < 3; ++r) evaluate(pb, fp32, sd);
+    HIPCHK(hipEventRecord(e1, pb.s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    ms /= 3;
+    if (verbose)
+      std::fprintf(stderr, "rism3d: tune %s planes %d zorder %d zwide %d nyq %d: %.3f ms\n",
+                   key.c_str(), c.planes, c.zorder, c.zwide, c.nyq, ms);
+    if (ms < best_ms) { best_ms = ms; best = c; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipGetLastError());
+  pb.knob[fp32 ? 1 : 0] = best;
+  std::lock_guard<std::mutex> lk(T.mu);
+  T.best[key] = {best, best_ms};
+  if (file)
+    if (FILE* f = std::fopen(file, "a")) {
+      std::fprintf(f, "%s %d %d %d %d %.4f\n", key.c_str(), best.planes, best.zorder, best.zwide,
+                   best.nyq, best_ms);
+      std::fclose(f);
+    }
 }
 
 struct Config {
@@ -2290,6 +2435,14 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     HIPCHK(hipGetLastError());
     res[0].status = RISM3D_OK;
     return RISM3D_OK;
+  }
+
+  // autotuner: the launch configuration of each working precision, on
+  // this batch (t, the history and the spectra are not read after it
+  // before the iteration writes them)
+  if (in.opt.tune && !in.noalloc) {
+    if (pb.mixed) tune(pb, true, in.opt.verbose);
+    tune(pb, false, in.opt.verbose);
   }
 
   Config cf;

@@ -68,7 +68,7 @@ class Options(ctypes.Structure):
                 ("continuation", ctypes.c_int),
                 ("chi3d", ctypes.POINTER(ctypes.c_double)),
                 ("site_rho", ctypes.POINTER(ctypes.c_double)),
-                ("coarse_tol", ctypes.c_double)]
+                ("coarse_tol", ctypes.c_double), ("tune", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -242,7 +242,8 @@ def make_options(settings=None, **kw):
                        ("mixed-tol", "mixed_tol"), ("diis-depth", "diis_depth"),
                        ("diis-step", "diis_step"), ("verbose", "verbose"),
                        ("device", "device"), ("dielectric", "dielectric"),
-                       ("continuation", "continuation"), ("coarse-tol", "coarse_tol")):
+                       ("continuation", "continuation"), ("coarse-tol", "coarse_tol"),
+                       ("tune", "tune")):
         if key in s:
             v = s[key]
             if key == "precision" and isinstance(v, str):

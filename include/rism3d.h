@@ -159,6 +159,18 @@ typedef struct rism3d_options {
                           N only.  Applies when every N[d] / 2 is a
                           supported size and neither chi3d nor a restart
                           token is given; otherwise ignored.               */
+  int tune;            /* 0: the built-in launch configuration.  1 (no
+                          counterpart in the reference): before iterating,
+                          time the candidate configurations of the residual
+                          chain on this batch (FP32 chain on xy planes or in
+                          five launches, z-pass block order, z-pass tile of
+                          16 or 32 kx columns, the Nyquist column in a
+                          launch of its own or not) and use the fastest;
+                          choices are kept per process by grid, sites,
+                          batch size and precision, and appended to the
+                          file RISM3D_TUNE_FILE names, if any.  Only the
+                          launch configuration changes (in FP32 the planes
+                          change the rounding order, not the fixed point)  */
 } rism3d_options;
 
 typedef struct rism3d_result {

@@ -34,3 +34,35 @@ def test_grids_equal_batches(chi_w, precision):
             assert out[i]["status"] == "OK" and r["status"] == "OK"
             assert out[i]["free-energy"] == r["free-energy"]
             assert out[i]["iterations"] == r["iterations"]
+
+
+def test_tuner_records_choices_and_keeps_results(chi_w, tmp_path, monkeypatch):
+    """options.tune: each working precision's launch configuration timed on
+    the batch and the fastest kept.  FP64 candidates differ only in the
+    z-pass block order (the same arithmetic per point: bit-identical mu);
+    mixed ones also in the FP32 chain layout (another rounding order, the
+    same fixed point).  The choices are appended to RISM3D_TUNE_FILE."""
+    print("case tuner", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd import synthetic as S
+    f = tmp_path / "tune.txt"
+    monkeypatch.setenv("RISM3D_TUNE_FILE", str(f))
+    sols = [S.solute(s) for s in range(4)]
+    base = dict(README, N=64, L=10.0, rmax=40.0, nrad=1536, closure="KH",
+                **{"norm-tol": 1e-9, "max-iter": 3000})
+    a = R.batch_solve(sols, W, base, chi_w)
+    b = R.batch_solve(sols, W, dict(base, tune=1), chi_w)
+    for x, y in zip(a, b):
+        assert x["status"] == "OK" and y["status"] == "OK"
+        assert x["free-energy"] == y["free-energy"]
+    am = R.batch_solve(sols, W, dict(base, precision="mixed"), chi_w)
+    bm = R.batch_solve(sols, W, dict(base, precision="mixed", tune=1), chi_w)
+    for x, y, z in zip(am, bm, a):
+        print(f"  mixed {x['free-energy']:.8f} tuned {y['free-energy']:.8f} fp64 {z['free-energy']:.8f}",
+              flush=True)
+        assert x["status"] == "OK" and y["status"] == "OK"
+        assert abs(y["free-energy"] - z["free-energy"]) < 0.01
+    lines = f.read_text().split("\n")
+    print("  " + "\n  ".join(lines), flush=True)
+    keys = [ln.split()[0] for ln in lines if ln.strip()]
+    assert "64x64x64,m3,B4,fp64" in keys and "64x64x64,m3,B4,fp32" in keys
