@@ -36,11 +36,11 @@ W = "water, PR-SPC/E"
 
 
 def settings_for(L_full, precision, depth=8, step=0.5, polish=3, mixed_tol=1e-5, N=128,
-                 coarse_tol=0.0):
+                 coarse_tol=0.0, tune=1):
     s = {"N": N, "L": L_full / 2, "rho": 0.0333295, "beta": 1.6889,
          "rmax": 160.0, "nrad": 4096, "closure": "KH", "max-iter": 3000,
          "norm-tol": 1e-10, "diis-depth": depth, "diis-step": step, "coarse-tol": coarse_tol,
-         "tune": 1}
+         "tune": tune}
     if precision == "mixed":
         s.update({"precision": "mixed", "mixed-tol": mixed_tol, "polish-iters": polish})
     return s
@@ -155,7 +155,8 @@ def natural_grids(solutes, chi, args):
         for N, idx, members, box in plan:
             for i in idx:
                 st[i] = settings_for(box, precision, args.diis_depth, args.diis_step, args.polish,
-                                     args.mixed_tol, N=N, coarse_tol=args.coarse_tol)
+                                     args.mixed_tol, N=N, coarse_tol=args.coarse_tol,
+                                     tune=0 if args.no_tune else 1)
         return R.batch_solve_grids(solutes, W, st, chi)
 
     run("mixed")
@@ -195,6 +196,9 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true",
                     help="time the mixed batch only (profiling runs: the FP64 comparison "
                          "batch and its fields are left out)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="the library's built-in launch configuration instead of the "
+                         "autotuner's (counter runs: the same kernels in every pass)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain elementwise copy of 512 MiB (b = 1.0 * a, FP64, "
                          "3 launches), the reference kernel for scaling FETCH_SIZE / "
@@ -235,10 +239,11 @@ def main():
     n = len(solutes)
     L_full = S.forced_box(solutes, N=128)
     chi = chi_solvent()
+    tune = 0 if args.no_tune else 1
     mixed = settings_for(L_full, "mixed", args.diis_depth, args.diis_step, args.polish,
-                         args.mixed_tol, coarse_tol=args.coarse_tol)
+                         args.mixed_tol, coarse_tol=args.coarse_tol, tune=tune)
     fp64 = settings_for(L_full, "fp64", args.diis_depth, args.diis_step,
-                        coarse_tol=args.coarse_tol)
+                        coarse_tol=args.coarse_tol, tune=tune)
     g_mixed = torch.empty((n, 3, 128, 128, 128), dtype=torch.float64, device=dev)
     ptr_mixed = [g_mixed[b].data_ptr() for b in range(n)]
     g_fp64 = None if args.skip_fp64 else torch.empty_like(g_mixed)
