@@ -61,8 +61,9 @@ __host__ __device__ constexpr int stage_ns(int n, int s) {  // product of earlie
 __host__ __device__ constexpr int lpos(int p) { return p; }
 __host__ __device__ constexpr int lstride_for(int n) { return n + 1; }
 
-template <typename T, int R, bool INV>
-__device__ __forceinline__ void dft_small(cpx<T> (&v)[4]) {
+template <typename T, int R, bool INV, int NV>
+__device__ __forceinline__ void dft_small(cpx<T> (&v)[NV]) {
+  static_assert(R <= NV, "radix larger than the points held");
   if constexpr (R == 2) {
     cpx<T> a = v[0], b = v[1];
     v[0] = cadd(a, b);
@@ -74,6 +75,27 @@ __device__ __forceinline__ void dft_small(cpx<T> (&v)[4]) {
     v[2] = csub(a0, a2);
     v[1] = cadd(a1, a3);
     v[3] = csub(a1, a3);
+  } else if constexpr (R == 8) {
+    // two 4-point DFTs of the even and odd points, then X_k = E_k + w^k O_k
+    // and X_{k+4} = E_k - w^k O_k with w = exp(-+2 pi i / 8)
+    cpx<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft_small<T, 4, INV>(e);
+    dft_small<T, 4, INV>(o);
+    const T h = T(0.70710678118654752440);  // 1/sqrt(2)
+    // w o_1 = (1 -+ i) o_1 / sqrt 2, w^2 o_2 = -+i o_2, w^3 o_3 = (-1 -+ i) o_3 / sqrt 2
+    const cpx<T> r1 = rot90<T, INV>(o[1]);
+    const cpx<T> w1 = {h * (o[1].re + r1.re), h * (o[1].im + r1.im)};
+    const cpx<T> w2 = rot90<T, INV>(o[2]);
+    const cpx<T> r3 = rot90<T, INV>(o[3]);
+    const cpx<T> w3 = {h * (r3.re - o[3].re), h * (r3.im - o[3].im)};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], w1);
+    v[5] = csub(e[1], w1);
+    v[2] = cadd(e[2], w2);
+    v[6] = csub(e[2], w2);
+    v[3] = cadd(e[3], w3);
+    v[7] = csub(e[3], w3);
   } else {  // R == 3
     const T h = T(0.86602540378443864676);  // sqrt(3)/2
     cpx<T> t1 = cadd(v[1], v[2]);
@@ -153,17 +175,14 @@ __device__ __forceinline__ void fft_lines(cpx<T>* buf, int lstride, int nl,
 // and out is a straight copy.  Butterflies are assigned line-fastest:
 // consecutive lanes touch consecutive LDS words on every read and on every
 // write after the first stage (whose writes are R-way strided).
-template <typename T, int N, int LINES, int NT, int STAGE, bool INV>
-__device__ __forceinline__ void stockham_stage_il(cpx<T>* buf, int nl,
-                                                  const cpx<T>* __restrict__ tw) {
+template <typename T, int N, int LINES, int NT, int R, int NS, bool INV>
+__device__ __forceinline__ void il_stage(cpx<T>* buf, int nl, const cpx<T>* __restrict__ tw) {
   static_assert((LINES & (LINES - 1)) == 0, "LINES must be a power of two");
-  constexpr int R = radix_at(N, STAGE);
-  constexpr int NS = stage_ns(N, STAGE);
   constexpr int NB = N / R;
   constexpr int BF = LINES * NB;
   constexpr int PER = (BF + NT - 1) / NT;
   constexpr int TWS = N / (NS * R);
-  cpx<T> v[PER][4];
+  cpx<T> v[PER][R];
   const int tid = threadIdx.x;
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
@@ -200,12 +219,107 @@ __device__ __forceinline__ void stockham_stage_il(cpx<T>* buf, int nl,
   __syncthreads();
 }
 
+template <typename T, int N, int LINES, int NT, int STAGE, bool INV>
+__device__ __forceinline__ void stockham_stage_il(cpx<T>* buf, int nl,
+                                                  const cpx<T>* __restrict__ tw) {
+  il_stage<T, N, LINES, NT, radix_at(N, STAGE), stage_ns(N, STAGE), INV>(buf, nl, tw);
+}
+
 template <typename T, int N, int LINES, int NT, bool INV, int S = 0>
 __device__ __forceinline__ void fft_lines_il(cpx<T>* buf, int nl, const cpx<T>* __restrict__ tw) {
   if constexpr (S < num_stages(N)) {
     stockham_stage_il<T, N, LINES, NT, S, INV>(buf, nl, tw);
     fft_lines_il<T, N, LINES, NT, INV, S + 1>(buf, nl, tw);
   }
+}
+
+// The interleaved FFT with the data held in registers on entry and exit.
+// A thread holds R1 = LINES * N / NT points of one line l = tid % LINES,
+// v[q] = point j0 + q * N / R1 with j0 = tid / LINES: the inputs of one
+// radix-R1 butterfly of the first Stockham stage, which therefore runs on
+// the registers and only writes LDS.  The later radices (8s while more than
+// R1 points remain, the last one at most R1) run through LDS; the last
+// stage reads LDS and keeps its outputs, which are again the points
+// j0 + q * N / R1 of the thread's line (its R1 / RL butterflies cover
+// exactly those), so v is overwritten in place with the transform.  Two LDS
+// passes per stage after the first, against one copy in, two per stage and
+// one copy out for fft_lines_il.
+__host__ __device__ constexpr int rr_radix(int n, int r1, int s) {  // stage s >= 1
+  int rem = n / r1;
+  for (int i = 1;; ++i) {
+    int r;
+    if (rem <= r1) r = rem;
+    else r = rem % 8 == 0 && rem > 8 ? 8 : (rem % 4 == 0 && rem > 4 ? 4 : 2);
+    if (i == s) return r;
+    rem /= r;
+    if (rem == 1) return 0;
+  }
+}
+__host__ __device__ constexpr int rr_stages(int n, int r1) {
+  int k = 1;
+  while (rr_radix(n, r1, k) != 0) ++k;
+  return k;
+}
+__host__ __device__ constexpr int rr_ns(int n, int r1, int s) {
+  int p = r1;
+  for (int i = 1; i < s; ++i) p *= rr_radix(n, r1, i);
+  return p;
+}
+// Whether an N-point transform of LINES lines over NT threads keeps its
+// points in registers: N a power of two, R1 = 4 or 8 points per thread
+// exactly.  (R1 = 2 is supported by the index scheme but needs one stage
+// more than fft_lines_il's radix-4 plan and more registers than it saves.)
+__host__ __device__ constexpr bool rr_ok(int n, int lines, int nt) {
+  if ((n & (n - 1)) != 0 || (lines * n) % nt != 0) return false;
+  const int r1 = lines * n / nt;
+  return (r1 == 4 || r1 == 8) && n > r1 && (nt % lines) == 0;
+}
+
+template <typename T, int N, int LINES, int NT, int R1, int S, bool INV>
+__device__ __forceinline__ void rr_mid(cpx<T>* buf, int nl, const cpx<T>* __restrict__ tw) {
+  if constexpr (S < rr_stages(N, R1) - 1) {
+    il_stage<T, N, LINES, NT, rr_radix(N, R1, S), rr_ns(N, R1, S), INV>(buf, nl, tw);
+    rr_mid<T, N, LINES, NT, R1, S + 1, INV>(buf, nl, tw);
+  }
+}
+
+template <typename T, int N, int LINES, int NT, bool INV>
+__device__ __forceinline__ void fft_lines_rr(cpx<T> (&v)[LINES * N / NT], cpx<T>* buf, int nl,
+                                             const cpx<T>* __restrict__ tw) {
+  constexpr int R1 = LINES * N / NT;
+  static_assert(rr_ok(N, LINES, NT), "points per thread must be one radix-2, 4 or 8 butterfly");
+  constexpr int S = rr_stages(N, R1);
+  constexpr int RL = rr_radix(N, R1, S - 1);
+  static_assert(RL >= 2 && R1 % RL == 0, "last radix must divide the first");
+  const int tid = threadIdx.x;
+  const int line = tid & (LINES - 1), j0 = tid / LINES;
+  // stage 0 (NS = 1, no twiddles): outputs to points j0 R1 + q
+  dft_small<T, R1, INV>(v);
+  if (line < nl) {
+#pragma unroll
+    for (int q = 0; q < R1; ++q) buf[(j0 * R1 + q) * LINES + line] = v[q];
+  }
+  __syncthreads();
+  rr_mid<T, N, LINES, NT, R1, 1, INV>(buf, nl, tw);
+  // last stage (NS = N / RL): butterfly j = j0 + p N / R1 reads points
+  // j + q N / RL, writes them back to the same points
+  constexpr int NBL = N / RL;
+#pragma unroll
+  for (int p = 0; p < R1 / RL; ++p) {
+    const int j = j0 + p * (N / R1);
+    cpx<T> u[RL];
+#pragma unroll
+    for (int q = 0; q < RL; ++q) u[q] = buf[(j + q * NBL) * LINES + line];
+#pragma unroll
+    for (int q = 1; q < RL; ++q) {
+      const cpx<T> w = tw[q * j];
+      u[q] = INV ? cmulc(u[q], w) : cmul(u[q], w);
+    }
+    dft_small<T, RL, INV>(u);
+#pragma unroll
+    for (int q = 0; q < RL; ++q) v[p + q * (R1 / RL)] = u[q];
+  }
+  __syncthreads();  // the buffer is free for the next transform
 }
 
 // The FFT of the NCOL columns of a row-major plane in LDS: point p of

@@ -130,6 +130,17 @@ __device__ __forceinline__ void stbf4(uint16_t* p, BF4 q) {
   *(G*)p = x;
 }
 
+// Index of the k point (kz, ky, kx) in a solute's form factor F(k) s(k):
+// z-major rows of NH like the tables in FP64; in FP32 y-major rows of
+// N0/2 bins (the layout of the plane chain's spectra) and the Nyquist
+// column kx = N0/2 compact behind them, so that the z pass reads it
+// as it reads the spectra.
+template <typename T>
+__device__ __forceinline__ int64_t fs_index(const GridArgs& g, int z, int y, int x) {
+  if constexpr (sizeof(T) == 8) return ((int64_t)z * g.N1 + y) * g.NH + x;
+  return x == g.NX - 1 ? g.K + (int64_t)z * g.N1 + y : ((int64_t)y * g.N2 + z) * g.NH + x;
+}
+
 // One vector of VW points at e (an offset into the slab off of solute d's
 // site arrays): the MDIIS step t = sum_j coef_j u_j (written to t) or the
 // given t, then c = closure(beta v, t) in the solver mode of d.
@@ -519,12 +530,12 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
         }
     }
   } else {
-  const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + base;
+  const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs);
   for (int e = threadIdx.x; e < N2 * LINES; e += NT) {
     const int z = e / LINES, l = e - z * LINES;
     if (kx0 + l >= g.NX) continue;
     const int64_t kk = base + z * zs + l;   // k-grid index of this point
-    const cpx<T> fs = Fs[z * zs + l];
+    const cpx<T> fs = Fs[fs_index<T>(g, z, y, kx0 + l)];
     cpx<T> c[MAX_SITES];
     for (int s = 0; s < m; ++s) c[s] = lds[(s * LINES + l) * LS + lpos(z)];
     cpx<T> uc = {T(0), T(0)};
@@ -614,6 +625,11 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
   // Block order.  zorder 2 (1D grid): consecutive blocks go to the XCDs in
   // turn, and each XCD walks all solutes of one (ky, kx tile) before the
   // next, so the tile's chi and tau stay in that XCD's L2 for every solute.
+  // zorder 3 (1D grid): XCD x owns the rows ky = x (mod 8) and walks every
+  // solute of a row, the row's kx tiles back to back, so each z line's
+  // columns are read by neighbouring workgroups at once (whole rows of the
+  // z planes instead of one tile's 128 bytes) and the row's chi and tau
+  // stay in the XCD's L2 across solutes.
   // zorder 1: x = kx tile, y = ky, z = solute; zorder 0: x = solute,
   // y = kx tile, z = ky.
   int b, tile, ky;
@@ -621,6 +637,11 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
     b = blockIdx.z;
     tile = 0;
     ky = blockIdx.x * LINES;
+  } else if (g.zorder == 3) {
+    const int lin = blockIdx.x, xcd = lin & 7, slot = lin >> 3;
+    tile = slot % g.ztiles;
+    b = (slot / g.ztiles) % g.zB;
+    ky = (slot / (g.ztiles * g.zB)) * 8 + xcd;
   } else if (g.zorder == 2) {
     const int lin = blockIdx.x, xcd = lin & 7, slot = lin >> 3;
     const int gt = (slot / g.zB) * 8 + xcd;  // global tile, ky-major
@@ -643,10 +664,23 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
   // kx0 + l of row y.
   const int y = ky;
   const int kx0 = NYQ ? g.NX - 1 : tile * LINES;
-  const int64_t zs = (int64_t)g.N1 * g.NH;  // z stride
+  const int64_t zs = (int64_t)g.N1 * g.NH;  // z stride (Fs, chi, tau)
   const int64_t ls = NYQ ? g.NH : 1;        // line stride
   const int64_t base = (int64_t)y * g.NH + kx0;
+  // W: the same index unless ylay (y-major rows, compact Nyquist column)
+  int64_t wz = zs, wl = ls, wsite = g.K;
   cpx<T>* W = static_cast<cpx<T>*>(d.W) + base;
+  if (g.ylay) {
+    if (NYQ) {
+      W = static_cast<cpx<T>*>(d.W) + g.m * g.K + y;
+      wz = g.N1;
+      wl = 1;
+      wsite = (int64_t)g.N1 * g.N2;
+    } else {
+      W = static_cast<cpx<T>*>(d.W) + (int64_t)y * g.N2 * g.NH + kx0;
+      wz = g.NH;
+    }
+  }
   const int nl = NYQ ? min(LINES, g.N1 - y) : min(LINES, g.NX - kx0);  // valid lines
   const int tid = threadIdx.x;
   cpx<T> c[MS][PPT];
@@ -660,21 +694,35 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
       const int e = tid + p * NTZ;
       if (PPT * NTZ == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
-        if (nl == LINES || l < nl) c[s][p] = ldc(W + s * g.K + z * zs + l * ls);
+        c[s][p] = (nl == LINES || l < nl) ? ldc(W + s * wsite + z * wz + l * wl)
+                                          : cpx<T>{T(0), T(0)};
       }
     }
   }
+  // RR: the z transforms keep the thread's points in registers (its PPT
+  // points of line l are the inputs of one first-stage butterfly and the
+  // outputs of its last-stage butterflies, fft_lines_rr); otherwise they
+  // pass through LDS whole (fft_lines_il).
+  constexpr bool RR = rr_ok(N2, LINES, NTZ) && PPT * NTZ == PTS;
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
-    cpx<T>* Ws = W + s * g.K;
+    cpx<T>* Ws = W + s * wsite;
+    if constexpr (RR) {
+      if (d.mode == 4) continue;  // inverse only: the loaded spectrum goes to the inverse pass
+      fft_lines_rr<T, N2, LINES, NTZ, false>(c[s], lds, nl, tw);
+      if (d.mode == 3) {  // forward spectrum only
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+          const int e = tid + p * NTZ;
+          const int z = e / LINES, l = e - z * LINES;
+          if (l < nl) stc(Ws + z * wz + l * wl, c[s][p]);
+        }
+      }
+      continue;
+    }
     {
       cpx<T>* v = c[s];
-      if (d.mode == 4) {  // inverse only: the loaded spectrum goes to the inverse pass
-#pragma unroll
-        for (int p = 0; p < PPT; ++p)
-          if ((tid + p * NTZ) % LINES >= nl) v[p] = cpx<T>{T(0), T(0)};
-        continue;
-      }
+      if (d.mode == 4) continue;  // inverse only: the loaded spectrum goes to the inverse pass
 #pragma unroll
       for (int p = 0; p < PPT; ++p) {
         const int e = tid + p * NTZ;
@@ -689,14 +737,14 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
       if (PPT * NTZ == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
         c[s][p] = lds[e];
-        if (d.mode == 3 && l < nl) stc(Ws + z * zs + l * ls, c[s][p]);  // forward spectrum only
+        if (d.mode == 3 && l < nl) stc(Ws + z * wz + l * wl, c[s][p]);  // forward spectrum only
       }
     }
     __syncthreads();
   }
   if (d.mode == 3) return;
   const bool linear = d.mode != 0;  // chi convolution only
-  const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs) + base;
+  const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs);
 #pragma unroll
   for (int p = 0; p < PPT; ++p) {
     const int e = tid + p * NTZ;
@@ -705,7 +753,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
     if (l >= nl) continue;
     const int64_t kk = base + z * zs + l * ls;  // k-grid index of this point
     cpx<T> fs = {T(0), T(0)};
-    if (!linear) fs = ldc(Fs + z * zs + l * ls);
+    if (!linear) fs = ldc(Fs + fs_index<T>(g, z, NYQ ? y + l : y, kx0 + (NYQ ? 0 : l)));
     cpx<T> uc = {T(0), T(0)};
     if (!g.renorm && !linear) {
       // u_c(k) = 4 pi eps0^-1 exp(-k^2/4G^2)/k^2 F(k) s(k), 0 at k = 0
@@ -750,7 +798,17 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
   }
 #pragma unroll
   for (int s = 0; s < MS; ++s) {
-    cpx<T>* Ws = W + s * g.K;
+    cpx<T>* Ws = W + s * wsite;
+    if constexpr (RR) {
+      fft_lines_rr<T, N2, LINES, NTZ, true>(c[s], lds, nl, tw);
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        const int e = tid + p * NTZ;
+        const int z = e / LINES, l = e - z * LINES;
+        if (l < nl) stc(Ws + z * wz + l * wl, c[s][p]);
+      }
+      continue;
+    }
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
       const int e = tid + p * NTZ;
@@ -766,7 +824,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
       const int e = tid + p * NTZ;
       if (PPT * NTZ == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
-        if (l < nl) stc(Ws + z * zs + l * ls, lds[e]);
+        if (l < nl) stc(Ws + z * wz + l * wl, lds[e]);
       }
     }
     __syncthreads();
@@ -1060,11 +1118,15 @@ closure_r2c_xy(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __re
   }
   __syncthreads();
   fft_cols<T, N1, LS, M + 1, NTP, false>(lds, tw_y);
-  cpx<T>* W = static_cast<cpx<T>*>(d.W) + site * g.K + (int64_t)z * N1 * g.NH;
-  for (int e = threadIdx.x; e < N1 * (M + 1); e += NTP) {
-    const int y = e / (M + 1), k = e - y * (M + 1);
-    stc(W + y * g.NH + k, lds[y * LS + k]);
+  // y-major rows of M bins, then the Nyquist bins of the plane's rows
+  cpx<T>* W = static_cast<cpx<T>*>(d.W) + site * g.K + (int64_t)z * g.NH;
+  const int64_t ys = (int64_t)g.N2 * g.NH;
+  for (int e = threadIdx.x; e < N1 * M; e += NTP) {
+    const int y = e / M, k = e - y * M;
+    stc(W + y * ys + k, lds[y * LS + k]);
   }
+  cpx<T>* Q = static_cast<cpx<T>*>(d.W) + g.m * g.K + ((int64_t)site * g.N2 + z) * N1;
+  for (int y = threadIdx.x; y < N1; y += NTP) stc(Q + y, lds[y * LS + M]);
 }
 
 // The inverse: the plane's rows [y][kx] from W, the inverse c2c of every
@@ -1087,25 +1149,39 @@ c2r_xy_resid(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __rest
   const SoluteDesc& d = sd[blockIdx.z];
   if (!d.active) return;
   const int site = blockIdx.y, z = blockIdx.x;
-  const cpx<T>* W = static_cast<const cpx<T>*>(d.W) + site * g.K + (int64_t)z * N1 * g.NH;
+  // the plane's y-major rows of M bins and its Nyquist bins (closure_r2c_xy)
+  const cpx<T>* W = static_cast<const cpx<T>*>(d.W) + site * g.K + (int64_t)z * g.NH;
+  const cpx<T>* Q = static_cast<const cpx<T>*>(d.W) + g.m * g.K + ((int64_t)site * g.N2 + z) * N1;
+  const int64_t ys = (int64_t)g.N2 * g.NH;
   {
-    constexpr int PTS = N1 * (M + 1), PT = (PTS + NTP - 1) / NTP;
-    cpx<T> v[PT];
+    constexpr int PTS = N1 * M, PT = (PTS + NTP - 1) / NTP;
+    constexpr int QT = (N1 + NTP - 1) / NTP;
+    cpx<T> v[PT], nq[QT];
 #pragma unroll
     for (int p = 0; p < PT; ++p) {  // all loads first
       const int e = threadIdx.x + p * NTP;
       if (PT * NTP == PTS || e < PTS) {
-        const int y = e / (M + 1), k = e - y * (M + 1);
-        v[p] = ldc(W + y * g.NH + k);
+        const int y = e / M, k = e - y * M;
+        v[p] = ldc(W + y * ys + k);
       }
+    }
+#pragma unroll
+    for (int p = 0; p < QT; ++p) {
+      const int y = threadIdx.x + p * NTP;
+      if (y < N1) nq[p] = ldc(Q + y);
     }
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
       const int e = threadIdx.x + p * NTP;
       if (PT * NTP == PTS || e < PTS) {
-        const int y = e / (M + 1), k = e - y * (M + 1);
+        const int y = e / M, k = e - y * M;
         lds[y * LS + k] = v[p];
       }
+    }
+#pragma unroll
+    for (int p = 0; p < QT; ++p) {
+      const int y = threadIdx.x + p * NTP;
+      if (y < N1) lds[y * LS + M] = nq[p];
     }
   }
   __syncthreads();
@@ -1500,7 +1576,9 @@ form_factor(const SitesDesc* __restrict__ sites, GridArgs g, cpx<double>* const*
     }
     const double sign = ((x + iy + iz) & 1) ? -1.0 : 1.0;
     fs64[blockIdx.y][e] = cpx<double>{sign * re, sign * im};
-    if (fs32) fs32[blockIdx.y][e] = cpx<float>{(float)(sign * re), (float)(sign * im)};
+    if (fs32)
+      fs32[blockIdx.y][fs_index<float>(g, z, y, x)] =
+          cpx<float>{(float)(sign * re), (float)(sign * im)};
   }
 }
 

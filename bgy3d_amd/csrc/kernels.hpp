@@ -72,6 +72,13 @@ struct GridArgs {
   int zB, ztiles;      // zorder 2: solutes and kx tiles of the launch
   int nyq;             // FP32 z pass: the Nyquist column in a launch of its own
   int zwide;           // FP32 z pass at N2 <= 128: 32 kx columns, 512 threads
+  // ylay (the FP32 xy-plane chain): W is y-major, point (site, ky, z, kx <
+  // N0/2) at site K + ky N2 NH + z NH + kx, so that the z pass reads whole
+  // regions of the spectrum, and the Nyquist column kx = N0/2 is compact
+  // behind the sites, (site, z, ky) at m K + site N1 N2 + z N1 + ky (cpx<T>
+  // units).  0: W is z-major like the tables, [site][z][ky][NH].
+  int ylay;
+  int pad_ylay_;
   double q[MAX_SITES]; // solvent charges
   // zstar variants (0 solute residual): 1 solvent-solvent OZ on site-site
   // pairs (msol sites, pair p = i(i+1)/2 + j), 2 Poisson potential of a

@@ -132,7 +132,7 @@ void launch_zstar_l(int N2, dim3 grid, size_t lds, hipStream_t s, const SoluteDe
   }
 }
 // Tuning knobs, read once: RISM3D_ZSTAR_LINES (kx columns per zstar
-// workgroup, 2..16) and RISM3D_ZSTAR_ORDER (0, 1 or 2, see zstar_reg).
+// workgroup, 2..16) and RISM3D_ZSTAR_ORDER (0..3, see zstar_reg).
 int env_int(const char* name, int dflt) {
   const char* v = std::getenv(name);
   return v ? std::atoi(v) : dflt;
@@ -161,12 +161,17 @@ void launch_zreg_n(int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd, c
   // separate launch cost more than the tile it replaced.  RISM3D_NYQ=0
   // keeps it in the last tile (g.nyq).
   const bool split = g.nyq && sizeof(T) == 4 && NX > 1 && (NX - 1) % L == 0;
+  if (g.ylay && !split) {  // evaluate forces the split for the plane chain
+    std::fprintf(stderr, "rism3d: y-major spectra need the Nyquist launch\n");
+    throw HipFailure{hipErrorInvalidValue};
+  }
   const int tiles = split ? (NX - 1) / L : (NX + L - 1) / L;
   GridArgs gz = g;
   if (gz.zorder == 2 && (tiles * N1) % 8 != 0) gz.zorder = 1;  // whole tiles per XCD
+  if (gz.zorder == 3 && N1 % 8 != 0) gz.zorder = 1;             // whole rows per XCD
   gz.zB = B;
   gz.ztiles = tiles;
-  const dim3 grid = gz.zorder == 2 ? dim3(tiles * N1 * B)
+  const dim3 grid = gz.zorder >= 2 ? dim3(tiles * N1 * B)
                   : gz.zorder ? dim3(tiles, N1, B) : dim3(B, tiles, N1);
   const dim3 gn((N1 + L - 1) / L, 1, B);
   if (g.m == 3) {
@@ -181,10 +186,13 @@ void launch_zreg_n(int NX, int N1, int B, hipStream_t s, const SoluteDesc* sd, c
 // three launches, the y transforms inside the plane kernels) when the
 // planes are square with 64, 96 or 128 points per side; otherwise, and in
 // FP64 (a 128^2 plane would take 133 KB of LDS), the five-launch chain.
-// RISM3D_PLANES=0 keeps the five-launch chain in FP32 too.
+// RISM3D_PLANES=0 keeps the five-launch chain in FP32 too.  The plane
+// kernels write W y-major with the Nyquist column apart (GridArgs::ylay),
+// which only zstar_reg reads (3 or 5 solvent sites).
+bool use_zreg(const GridArgs& g);
 bool use_planes(const GridArgs& g, bool fp32, int on) {
   return on && fp32 && g.special == 0 && g.N0 == g.N1 &&
-         (g.N0 == 64 || g.N0 == 96 || g.N0 == 128);
+         (g.N0 == 64 || g.N0 == 96 || g.N0 == 128) && use_zreg(g);
 }
 template <typename T, int N, int H>
 void launch_planes_h(bool inv, dim3 grid, hipStream_t s, const SoluteDesc* sd, const GridArgs& g,
@@ -518,7 +526,7 @@ struct SoluteState {
   DevBuf<float> bv32;      // and FP32 (mixed only)
   DevBuf<cpx<double>> W;   // spectral work (FP32 phase uses the front half)
   DevBuf<cpx<double>> fs64;
-  DevBuf<cpx<float>> fs32;
+  DevBuf<cpx<float>> fs32;  // y-major with the Nyquist column apart (fs_index)
   DevBuf<double> part;     // residual-workgroup partials, NPART each
   DevBuf<double> sites;    // n * 6
   DevBuf<double> Dt, gmu;  // adjoint: dc/dt and dmu/dt at the converged t
@@ -559,6 +567,8 @@ struct Problem {
   int D;          // DIIS depth (1 = Picard)
   int mixed;
   int r16 = env_int("RISM3D_R_BF16", 1);  // FP32 phase: residual history in BF16
+  int D32 = env_int("RISM3D_DEPTH32", 0);   // FP32 phase: MDIIS depth (0: D)
+  double lam32 = std::getenv("RISM3D_STEP32") ? std::atof(std::getenv("RISM3D_STEP32")) : 0.0;
   int B;
   int64_t mP;
   hipStream_t s;
@@ -601,8 +611,8 @@ struct Problem {
     int zorder;  // z-pass block order (zstar_reg)
     int zwide;   // z-pass tile of 32 kx columns over 512 threads (FP32)
     int nyq;     // Nyquist column in a launch of its own (FP32)
-  } knob[2] = {{0, env_int("RISM3D_ZSTAR_ORDER", 2), 0, 0},
-               {env_int("RISM3D_PLANES", 1), env_int("RISM3D_ZSTAR_ORDER", 2),
+  } knob[2] = {{0, env_int("RISM3D_ZSTAR_ORDER", 3), 0, 0},
+               {env_int("RISM3D_PLANES", 1), env_int("RISM3D_ZSTAR_ORDER", 3),
                 env_int("RISM3D_ZSTAR_WIDE", 0), env_int("RISM3D_NYQ", 1)}};
   int noalloc = 0;  // buffers from the fixed scratch: no pinned staging, no graphs
 };
@@ -664,7 +674,7 @@ int hist_of(const std::vector<SoluteDesc>& sd) {
 void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
   GridArgs g = pb.g;
   const Problem::Knobs& kn = pb.knob[fp32 ? 1 : 0];
-  g.zorder = std::min(2, std::max(0, kn.zorder));
+  g.zorder = std::min(3, std::max(0, kn.zorder));
   g.zwide = kn.zwide;
   g.nyq = kn.nyq;
   upload_sd(pb, sd);
@@ -675,6 +685,13 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
   };
   const int hist = hist_of(sd);
   const bool planes = use_planes(g, fp32, kn.planes);
+  if (planes) {
+    // y-major spectra, the Nyquist column in the z pass's launch of its
+    // own (32-column tiles only where they end at it)
+    g.ylay = 1;
+    g.nyq = 1;
+    if (g.zwide && (g.NX - 1) % 32 != 0) g.zwide = 0;
+  }
   const int nparts = planes ? g.m * g.N2 : pb.nparts;  // partial rows per solute
   auto chain = [&]() {
     if (planes) {
@@ -811,10 +828,11 @@ void tune(Problem& pb, bool fp32, int verbose) {
   const bool can_planes = fp32 && use_planes(g0, true, 1);
   const bool can_wide = fp32 && g0.N2 <= 128 && use_zreg(g0);
   for (int planes : {1, 0})
-    for (int zorder : {2, 1})
+    for (int zorder : {3, 2, 1})
       for (int zwide : {0, 1})
         for (int nyq : {1, 0}) {
-          if ((planes && !can_planes) || (zwide && !can_wide) || (!fp32 && (nyq || zwide)))
+          if ((planes && !can_planes) || (zwide && !can_wide) || (!fp32 && (nyq || zwide)) ||
+              (planes && !nyq))  // the plane chain always splits the Nyquist column off
             continue;
           const Problem::Knobs& d0 = cand[0];
           if (planes == d0.planes && zorder == d0.zorder && zwide == d0.zwide && nyq == d0.nyq)
@@ -967,6 +985,9 @@ void rebase(Problem& pb, const Config& cf, const std::vector<int>& mask, bool ad
 // delta just folded into t_b becomes the shift of the entries of the old
 // base (t = sum_j c_j u_j - (sum_{old j} c_j) shift), entries of the base
 // before are dropped, and the coefficients are refitted if any was.
+// The FP32 phase's MDIIS step (RISM3D_STEP32, default the solve's step).
+double lam32(const Problem& pb, const Config& cf) { return pb.lam32 > 0 ? pb.lam32 : cf.lam; }
+
 void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
   auto& s = pb.st[b];
   const int64_t mP = pb.mP;
@@ -981,7 +1002,7 @@ void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
     s.sbase.assign(D, 0);
   }
   hipLaunchKernelGGL(rebase_prep, dim3(ch), dim3(NT), 0, pb.s, s.t64.p, s.bv64.p, s.r64.p,
-                     pb.g.closure == RISM3D_CLOSURE_PY ? 1 : 0, seed ? 1 : 0, (float)cf.lam,
+                     pb.g.closure == RISM3D_CLOSURE_PY ? 1 : 0, seed ? 1 : 0, (float)lam32(pb, cf),
                      s.bv32.p, s.fb32.p, s.dcur, U0, R0, pb.r16, mP);
   s.best32 = INFINITY;
   s.stall32 = 0;
@@ -1113,6 +1134,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
     // FP32 (the others wait), then FP64.
     const bool fp32 = any32;
     const size_t sz = fp32 ? 4 : 8;
+    const int Dp = fp32 && pb.D32 > 0 ? std::min(D, pb.D32) : D;  // this phase's depth
     for (int b = 0; b < B; ++b) {
       auto& s = pb.st[b];
       SoluteDesc& d = sd[b];
@@ -1132,7 +1154,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
       d.shift = s.dshift;
       // slot for the new r, u: a free one, else the oldest (dropped)
       s.dotted = s.order;
-      if ((int)s.order.size() < D) {
+      if ((int)s.order.size() < Dp) {
         int q = 0;
         while (std::find(s.order.begin(), s.order.end(), q) != s.order.end()) ++q;
         s.slot_new = q;
@@ -1144,7 +1166,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
       for (int j = 0; j < d.ndot; ++j) d.rh[j] = slotR(s.dotted[j]);
       d.r = slotR(s.slot_new);
       d.unew = slotU(s.slot_new);
-      d.lam = cf.lam;
+      d.lam = fp32 ? lam32(pb, cf) : cf.lam;
       d.bv = fp32 ? (const void*)s.bv32.p : (const void*)s.bv64.p;
       d.W = s.W.p;
       d.Fs = fp32 ? (const void*)s.fs32.p : (const void*)s.fs64.p;
@@ -1232,7 +1254,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
       s.gram[s.slot_new * D + s.slot_new] = r2;
       // restart when the residual blows up against the best one, or stalls
       if (r2 < s.best) { s.best = r2; s.since_best = 0; } else { s.since_best++; }
-      if (r2 > 100.0 * s.best || s.since_best > 3 * D) {
+      if (r2 > 100.0 * s.best || s.since_best > 3 * Dp) {
         s.order.assign(1, s.slot_new);
         s.best = r2;
         s.since_best = 0;
@@ -1968,7 +1990,7 @@ int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
   g.beta = pd->beta;
   g.rho = pd->rho;
   g.renorm = 0;
-  g.zorder = std::min(2, std::max(0, env_int("RISM3D_ZSTAR_ORDER", 2)));
+  g.zorder = std::min(3, std::max(0, env_int("RISM3D_ZSTAR_ORDER", 3)));
   g.closure = pd->closure;
   g.pse_order = pd->closure == RISM3D_CLOSURE_KH ? 1 : (pd->closure >= RISM3D_CLOSURE_PSE0 ? pd->closure - RISM3D_CLOSURE_PSE0 : 0);
   for (int i = 0; i < m; ++i) g.q[i] = solvent[i].charge;
@@ -2162,7 +2184,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   g.tau_scale = pd.beta * EPSILON0INV / L3;
   g.beta = pd.beta;
   g.renorm = (in.opt.renorm && !in.opt.chi3d) ? 1 : 0;
-  g.zorder = std::min(2, std::max(0, env_int("RISM3D_ZSTAR_ORDER", 2)));
+  g.zorder = std::min(3, std::max(0, env_int("RISM3D_ZSTAR_ORDER", 3)));
   g.closure = pd.closure;
   g.pse_order = pd.closure == RISM3D_CLOSURE_KH ? 1 : (pd.closure >= RISM3D_CLOSURE_PSE0 ? pd.closure - RISM3D_CLOSURE_PSE0 : 0);
   for (int i = 0; i < in.m; ++i) g.q[i] = in.solvent[i].charge;
@@ -2259,7 +2281,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     }
     s.W = DevBuf<cpx<double>>(mK);
     s.fs64 = DevBuf<cpx<double>>(g.K);
-    if (pb.mixed) s.fs32 = DevBuf<cpx<float>>(g.K);
+    if (pb.mixed) s.fs32 = DevBuf<cpx<float>>(g.K + (int64_t)g.N1 * g.N2);  // fs_index
     s.part = DevBuf<double>((size_t)pb.nparts * NPART);
     s.n_sites = in.n_sites[b];
     auto& sh = site_host[b];
@@ -2598,7 +2620,7 @@ size_t noalloc_bytes(const rism3d_problem& pd, int m, int n, bool mixed) {
   b += al(np * nrad * 8) + al(m * nrad * 8);                       // 1D tables
   b += al(np * K * 8) + al(np * K * 4) + al(m * K * 8) + al(m * K * 4);  // 3D tables
   b += 4 * al(mP * 8) + al(m * K * 16) + al(K * 16);  // U, R, t, beta v; W; F s
-  if (mixed) b += 4 * al(mP * 4) + al(mP * 8) + al(K * 8);
+  if (mixed) b += 4 * al(mP * 4) + al(mP * 8) + al((K + N1 * N2) * 8);
   b += al(N1 * N2 * m * NPART * 8) + al((size_t)std::max(n, 0) * 6 * 8);
   b += al(sizeof(SoluteDesc)) + al(sizeof(SitesDesc)) + al(NPART * 8) + 3 * al(8);
   b += al(3 * (size_t)m * 8) + 4 * al(8);  // setup descriptors
