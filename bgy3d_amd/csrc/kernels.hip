@@ -129,6 +129,12 @@ __device__ __forceinline__ void stbf4(uint16_t* p, BF4 q) {
   x[1] = q.hi;
   *(G*)p = x;
 }
+__device__ __forceinline__ BF4 to_bf4(const Vec16<float>& v) {
+  BF4 q;
+  q.lo = bf16_bits(v.v[0]) | (bf16_bits(v.v[1]) << 16);
+  q.hi = bf16_bits(v.v[2]) | (bf16_bits(v.v[3]) << 16);
+  return q;
+}
 
 // Index of the k point (kz, ky, kx) in a solute's form factor F(k) s(k):
 // z-major rows of NH like the tables in FP64; in FP32 y-major rows of
@@ -154,26 +160,50 @@ __device__ __forceinline__ Vec16<T> closure_vec(const SoluteDesc& d, const GridA
   Vec16<T> tv;
   const Vec16<T> b = ld16(bv + e);
   if (nc > 0) {
-    // all loads first, then the sums: independent loads in flight
-    Vec16<T> u[H];
-#pragma unroll
-    for (int j = 0; j < H; ++j)
-      if (j < nc) u[j] = ld16(static_cast<const T*>(d.u[j]) + off + e);
-    Vec16<T> sh;
-    if (d.shift_w != 0.0) sh = ld16(static_cast<const T*>(d.shift) + off + e);
     double acc[VW];
 #pragma unroll
     for (int k = 0; k < VW; ++k) acc[k] = 0.0;
+    bool summed = false;
+    if constexpr (sizeof(T) == 4) {
+      if (d.u16) {  // the newest u (FP32) and BF16 differences, loads first
+        const Vec16<T> u0 = ld16(static_cast<const T*>(d.u[0]) + off + e);
+        BF4 du[H];
 #pragma unroll
-    for (int j = 0; j < H; ++j)
-      if (j < nc) {
-        const double cj = d.coef[j];
+        for (int j = 1; j < H; ++j)
+          if (j < nc) du[j] = ldbf4(static_cast<const uint16_t*>(d.u[j]) + off + e);
 #pragma unroll
-        for (int k = 0; k < VW; ++k) acc[k] += cj * (double)u[j].v[k];
+        for (int k = 0; k < VW; ++k) acc[k] = d.coef[0] * (double)u0.v[k];
+#pragma unroll
+        for (int j = 1; j < H; ++j)
+          if (j < nc) {
+            const double cj = d.coef[j];
+            acc[0] += cj * (double)bf16_val(du[j].lo & 0xFFFFu);
+            acc[1] += cj * (double)bf16_val(du[j].lo >> 16);
+            acc[2] += cj * (double)bf16_val(du[j].hi & 0xFFFFu);
+            acc[3] += cj * (double)bf16_val(du[j].hi >> 16);
+          }
+        summed = true;
       }
-    if (d.shift_w != 0.0) {
+    }
+    if (!summed) {
+      // all loads first, then the sums: independent loads in flight
+      Vec16<T> u[H];
 #pragma unroll
-      for (int k = 0; k < VW; ++k) acc[k] -= d.shift_w * (double)sh.v[k];
+      for (int j = 0; j < H; ++j)
+        if (j < nc) u[j] = ld16(static_cast<const T*>(d.u[j]) + off + e);
+      Vec16<T> sh;
+      if (d.shift_w != 0.0) sh = ld16(static_cast<const T*>(d.shift) + off + e);
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+        if (j < nc) {
+          const double cj = d.coef[j];
+#pragma unroll
+          for (int k = 0; k < VW; ++k) acc[k] += cj * (double)u[j].v[k];
+        }
+      if (d.shift_w != 0.0) {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) acc[k] -= d.shift_w * (double)sh.v[k];
+      }
     }
 #pragma unroll
     for (int k = 0; k < VW; ++k) tv.v[k] = T(acc[k]);
@@ -949,7 +979,18 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
     }
 #pragma unroll
     for (int k = 0; k < VW; ++k) uv.v[k] = tv.v[k] + lam * rv.v[k];
-    if (un) st16(un + e, uv);
+    if (un) {
+      if constexpr (sizeof(T) == 4) {
+        if (d.u16) {  // the difference from the u it replaces, in BF16
+          const Vec16<T> uo = ld16(un + e);
+          Vec16<T> du;
+#pragma unroll
+          for (int k = 0; k < VW; ++k) du.v[k] = uv.v[k] - uo.v[k];
+          stbf4(static_cast<uint16_t*>(d.dnew) + off + e, to_bf4(du));
+        }
+      }
+      st16(un + e, uv);
+    }
 #pragma unroll
     for (int k = 0; k < VW; ++k) mx = fmax(mx, fabs((double)rv.v[k]));
     bool stored = false;
@@ -1223,6 +1264,9 @@ c2r_xy_resid(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __rest
   // one vector of VW points at e: t' from LDS, the streams t, F(t_b)
   // (mode 5) and the history r_j (rhj(j), unpacked on use)
   auto point = [&](int e, const Vec16<T>& tv, const Vec16<T>& f, auto&& rhj) {
+    Vec16<T> uo;  // u16: the u the new one replaces (loaded first, used last)
+    if constexpr (sizeof(T) == 4)
+      if (d.u16 && un) uo = ld16(un + e);
     Vec16<T> zv;  // t' at points e.. of row e / N0
     {
       const cpx<T>* zrow = lds + (e / N0) * LS;
@@ -1245,7 +1289,17 @@ c2r_xy_resid(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __rest
     }
 #pragma unroll
     for (int k = 0; k < VW; ++k) uv.v[k] = tv.v[k] + lam * rv.v[k];
-    if (un) st16(un + e, uv);
+    if (un) {
+      if constexpr (sizeof(T) == 4) {
+        if (d.u16) {  // the difference from the u it replaces (uo), in BF16
+          Vec16<T> du;
+#pragma unroll
+          for (int k = 0; k < VW; ++k) du.v[k] = uv.v[k] - uo.v[k];
+          stbf4(static_cast<uint16_t*>(d.dnew) + off + e, to_bf4(du));
+        }
+      }
+      st16(un + e, uv);
+    }
 #pragma unroll
     for (int k = 0; k < VW; ++k) mx = fmax(mx, fabs((double)rv.v[k]));
     bool stored = false;
@@ -1411,6 +1465,14 @@ add_delta(double* __restrict__ tb, const float* __restrict__ delta, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * NT;
   for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride)
     tb[e] += (double)delta[e];
+}
+
+// Defect correction, the newest u of a difference-form history moved to
+// the coordinates of a new base: u -= delta (the delta folded into t_b).
+__global__ void __launch_bounds__(NT)
+shift_u(float* __restrict__ u, const float* __restrict__ delta, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) u[e] -= delta[e];
 }
 
 // Defect correction, new base t_b with F(t_b) = f64 known: x_b = -beta v +

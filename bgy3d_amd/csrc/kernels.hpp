@@ -44,7 +44,12 @@ struct SoluteDesc {
   const void* shift;         // mode 5: t = sum_j coef_j u_j - shift_w shift: the
   double shift_w;            //   entries of the previous base moved to this one
   int r16;                   // FP32: r and the history rh in BF16 (Gram only)
-  int pad16_;
+  int u16;                   // FP32: u history in difference form: u[0] the newest
+                             //   u (FP32), u[j > 0] BF16 differences of consecutive
+                             //   entries, t = sum_j coef_j u[j]; the residual pass
+                             //   writes the new u over u[0] (unew) and its
+                             //   difference from the old one to dnew
+  void* dnew;                // u16: BF16 difference out
 };
 constexpr int NPART = 2 + MAX_DIIS;   // doubles per residual workgroup
 constexpr int INFO_NSUM = 22;         // grid sums of the info observables

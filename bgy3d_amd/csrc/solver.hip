@@ -536,6 +536,7 @@ struct SoluteState {
   float* dcur = nullptr;   // mixed: the correction delta of the current base
   float* dshift = nullptr; // mixed: the last delta folded into the base, by
                            // which entries of the previous base are moved
+  float* ucur = nullptr;   // mixed, difference-form history: the newest u
   int base = 0;            // mixed: rebases so far in this FP32 phase
   std::vector<int> sbase;  // mixed: base of each history slot
   double tol_lin = 0.0;    // adjoint: absolute tolerance (relative to |g|)
@@ -569,6 +570,9 @@ struct Problem {
   int r16 = env_int("RISM3D_R_BF16", 1);  // FP32 phase: residual history in BF16
   int D32 = env_int("RISM3D_DEPTH32", 0);   // FP32 phase: MDIIS depth (0: D)
   double lam32 = std::getenv("RISM3D_STEP32") ? std::atof(std::getenv("RISM3D_STEP32")) : 0.0;
+  // FP32 phase (with r16): the u history in difference form, the newest u
+  // in FP32 and the differences of consecutive entries in BF16
+  int u16 = env_int("RISM3D_U_DIFF", 1);
   int B;
   int64_t mP;
   hipStream_t s;
@@ -749,13 +753,13 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
       ++na;
       // u_j in (+ the shift of the previous base's entries), t out (or t
       // in), beta v in, C out
-      b0 += d.ncomb * mP * sc + (d.shift_w != 0.0 ? mP * sc : 0) + mP * sc + mP * sc +
-            mK * 2 * sc;
+      const double uin = d.u16 ? mP * sc + (d.ncomb - 1) * mP * 2.0 : d.ncomb * mP * sc;
+      b0 += uin + (d.shift_w != 0.0 ? mP * sc : 0) + mP * sc + mP * sc + mK * 2 * sc;
       // W in, t in, r and u out, r_j in (+ F(t_b) in, defect correction);
       // r and r_j in BF16 with r16
       const double sr = d.r16 ? 2.0 : sc;
       b4 += mK * 2 * sc + mP * sc + mP * sr + mP * sc + d.ndot * mP * sr +
-            (d.mode == 5 ? mP * sc : 0);
+            (d.mode == 5 ? mP * sc : 0) + (d.u16 ? mP * sc + mP * 2.0 : 0);
     }
     const double tables = (double)(g.m * (g.m + 1) / 2 + g.m) * Kv * sc;
     const double per[5] = {b0, na * 2 * mK * 2 * sc,
@@ -1000,6 +1004,11 @@ void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
     s.dshift = s.t32b.p;
     s.base = 0;
     s.sbase.assign(D, 0);
+    // difference form: D BF16 slots at the front of U, the newest u behind
+    s.ucur = pb.u16 ? reinterpret_cast<float*>(reinterpret_cast<char*>(s.U.p) +
+                                               (((size_t)D * mP * 2 + 255) & ~(size_t)255))
+                    : nullptr;
+    if (pb.u16) U0 = s.ucur;
   }
   hipLaunchKernelGGL(rebase_prep, dim3(ch), dim3(NT), 0, pb.s, s.t64.p, s.bv64.p, s.r64.p,
                      pb.g.closure == RISM3D_CLOSURE_PY ? 1 : 0, seed ? 1 : 0, (float)lam32(pb, cf),
@@ -1031,6 +1040,9 @@ void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
   }
   std::swap(s.dcur, s.dshift);
   s.base++;
+  // difference form: the newest u moves to the new base (differences of
+  // entries do not change), so no shift enters the later steps
+  if (pb.u16) hipLaunchKernelGGL(shift_u, dim3(ch), dim3(NT), 0, pb.s, s.ucur, s.dshift, mP);
 }
 
 // Runs the fixed-point iteration for every solute of the batch.  MDIIS
@@ -1146,10 +1158,26 @@ void iterate_batch(Problem& pb, const Config& cf) {
       auto slotR = [&](int q) { return (void*)((char*)s.R.p + (size_t)q * mP * szr); };
       d.t = fp32 ? (void*)s.dcur : (void*)s.t64.p;
       d.ncomb = (int)s.coef.size();
-      for (int j = 0; j < d.ncomb; ++j) {
-        d.u[j] = slotU(s.order[j]);
-        d.coef[j] = s.coef[j];
-        if (fp32 && s.sbase[s.order[j]] != s.base) d.shift_w += s.coef[j];
+      const bool u16 = fp32 && pb.u16;
+      auto slotD = [&](int q) { return (void*)((uint16_t*)s.U.p + (size_t)q * mP); };
+      if (u16) {
+        // t = sum_j c_j u_j = u_newest - sum_i (c_0 + .. + c_i) (u_i+1 - u_i),
+        // the difference u_i+1 - u_i in the BF16 slot of entry i + 1
+        d.u16 = 1;
+        d.u[0] = s.ucur;
+        d.coef[0] = 1.0;
+        double cum = 0.0;
+        for (int i = 0; i + 1 < d.ncomb; ++i) {
+          cum += s.coef[i];
+          d.u[1 + i] = slotD(s.order[i + 1]);
+          d.coef[1 + i] = -cum;
+        }
+      } else {
+        for (int j = 0; j < d.ncomb; ++j) {
+          d.u[j] = slotU(s.order[j]);
+          d.coef[j] = s.coef[j];
+          if (fp32 && s.sbase[s.order[j]] != s.base) d.shift_w += s.coef[j];
+        }
       }
       d.shift = s.dshift;
       // slot for the new r, u: a free one, else the oldest (dropped)
@@ -1165,7 +1193,8 @@ void iterate_batch(Problem& pb, const Config& cf) {
       d.ndot = (int)s.dotted.size();
       for (int j = 0; j < d.ndot; ++j) d.rh[j] = slotR(s.dotted[j]);
       d.r = slotR(s.slot_new);
-      d.unew = slotU(s.slot_new);
+      d.unew = u16 ? (void*)s.ucur : slotU(s.slot_new);
+      d.dnew = u16 ? slotD(s.slot_new) : nullptr;
       d.lam = fp32 ? lam32(pb, cf) : cf.lam;
       d.bv = fp32 ? (const void*)s.bv32.p : (const void*)s.bv64.p;
       d.W = s.W.p;
@@ -2155,6 +2184,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   // BF16 residual history with defect correction only: the FP32-only study
   // iterates t itself to its floor, where the Gram rounding would show
   if (in.opt.precision != RISM3D_MIXED) pb.r16 = 0;
+  if (!pb.r16) pb.u16 = 0;
   for (int d = 0; d < 3; ++d) { pb.N[d] = pd.N[d]; pb.L[d] = pd.L[d]; pb.h[d] = pd.L[d] / pd.N[d]; }
   GridArgs& g = pb.g;
   std::memset(&g, 0, sizeof g);
