@@ -472,44 +472,69 @@ void make_twiddles(Twiddles& tw, const int N[3], hipStream_t s) {
 
 // ------------------------------------------------------------ DIIS solve
 // Minimise |sum c_j r_j|^2 subject to sum c_j = 1 (Pulay).  G is the Gram
-// matrix over the nh history entries (in order).  Returns false if the
-// bordered system is singular.
+// matrix over the nh history entries (in order).  The bordered system
+// [G/s -1; -1 0] [c; l] = [0; -1] (s the largest diagonal entry) is solved
+// in FP64 by LU with partial pivoting and one step of iterative refinement
+// against the unfactored matrix, on the stack: it runs for every solute of
+// a batch between two residual passes, while the device waits.  Returns
+// false if the system is singular or the result not finite.
 bool diis_coefficients(const std::vector<double>& G, int nh, std::vector<double>& c) {
+  constexpr int NM = MAX_DIIS + 1;
   const int n = nh + 1;
-  std::vector<long double> A(n * n, 0.0L), b(n, 0.0L);
-  long double scale = 0.0L;
-  for (int i = 0; i < nh; ++i) scale = std::max(scale, (long double)std::fabs(G[i * nh + i]));
-  if (!(scale > 0.0L) || !std::isfinite((double)scale)) return false;
-  for (int i = 0; i < nh; ++i) {
-    for (int j = 0; j < nh; ++j) A[i * n + j] = G[i * nh + j] / scale;
-    A[i * n + nh] = -1.0L;
-    A[nh * n + i] = -1.0L;
+  if (nh < 1 || nh > MAX_DIIS) return false;
+  double scale = 0.0;
+  for (int i = 0; i < nh; ++i) scale = std::max(scale, std::fabs(G[i * nh + i]));
+  if (!(scale > 0.0) || !std::isfinite(scale)) return false;
+  double A0[NM][NM], A[NM][NM], b0[NM], x[NM], r[NM];
+  int perm[NM];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      A0[i][j] = (i < nh && j < nh) ? G[i * nh + j] / scale : (i == nh && j == nh ? 0.0 : -1.0);
+  for (int i = 0; i < n; ++i) {
+    b0[i] = i == nh ? -1.0 : 0.0;
+    perm[i] = i;
+    for (int j = 0; j < n; ++j) A[i][j] = A0[i][j];
   }
-  b[nh] = -1.0L;
-  for (int col = 0; col < n; ++col) {
+  for (int col = 0; col < n; ++col) {  // PA = LU in place, L unit lower
     int piv = col;
-    for (int r = col + 1; r < n; ++r)
-      if (std::fabs((double)A[r * n + col]) > std::fabs((double)A[piv * n + col])) piv = r;
-    if (std::fabs((double)A[piv * n + col]) < 1e-300) return false;
+    for (int rr = col + 1; rr < n; ++rr)
+      if (std::fabs(A[rr][col]) > std::fabs(A[piv][col])) piv = rr;
+    if (!(std::fabs(A[piv][col]) > 1e-300)) return false;
     if (piv != col) {
-      for (int k = 0; k < n; ++k) std::swap(A[col * n + k], A[piv * n + k]);
-      std::swap(b[col], b[piv]);
+      for (int k = 0; k < n; ++k) std::swap(A[col][k], A[piv][k]);
+      std::swap(perm[col], perm[piv]);
     }
-    for (int r = col + 1; r < n; ++r) {
-      const long double f = A[r * n + col] / A[col * n + col];
-      for (int k = col; k < n; ++k) A[r * n + k] -= f * A[col * n + k];
-      b[r] -= f * b[col];
+    const double inv = 1.0 / A[col][col];
+    for (int rr = col + 1; rr < n; ++rr) {
+      const double f = A[rr][col] * inv;
+      A[rr][col] = f;
+      for (int k = col + 1; k < n; ++k) A[rr][k] -= f * A[col][k];
     }
   }
-  std::vector<long double> x(n);
-  for (int r = n - 1; r >= 0; --r) {
-    long double s = b[r];
-    for (int k = r + 1; k < n; ++k) s -= A[r * n + k] * x[k];
-    x[r] = s / A[r * n + r];
+  auto lu_solve = [&](const double* rhs, double* out) {
+    double y[NM];
+    for (int i = 0; i < n; ++i) {
+      double t = rhs[perm[i]];
+      for (int k = 0; k < i; ++k) t -= A[i][k] * y[k];
+      y[i] = t;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double t = y[i];
+      for (int k = i + 1; k < n; ++k) t -= A[i][k] * out[k];
+      out[i] = t / A[i][i];
+    }
+  };
+  lu_solve(b0, x);
+  for (int i = 0; i < n; ++i) {  // one refinement step: x += A^-1 (b - A x)
+    double t = b0[i];
+    for (int k = 0; k < n; ++k) t -= A0[i][k] * x[k];
+    r[i] = t;
   }
+  double dx[NM];
+  lu_solve(r, dx);
   c.assign(nh, 0.0);
   for (int i = 0; i < nh; ++i) {
-    c[i] = (double)x[i];
+    c[i] = x[i] + dx[i];
     if (!std::isfinite(c[i])) return false;
   }
   return true;
