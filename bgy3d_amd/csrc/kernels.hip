@@ -388,6 +388,105 @@ ktab3(GridArgs g, const double* __restrict__ chi_tab, const double* __restrict__
   }
 }
 
+// The per-k solvent-solvent OZ of one k point (iterate_t2, hnc3d.c:376-470;
+// compute_t2_m, hnc3d.c:231-324) for g.msol <= MSX sites, the site-site
+// pair spectra of the point in LDS (pair p = i(i+1)/2 + j at
+// lds[(p LINES + l) LS + lpos(z)]), overwritten with T.  sgn is the corner
+// phase of the point, U the unit long-range Coulomb there, kk its index
+// into the omega table chi3.
+template <typename T, int MSX, int LINES, int LS>
+__device__ void oz_pairs(cpx<T>* __restrict__ lds, const GridArgs& g, const T* __restrict__ chi3,
+                         int64_t kk, int l, int z, double sgn, double U) {
+  // solvent-solvent OZ (iterate_t2, hnc3d.c:376-470; compute_t2_m,
+  // hnc3d.c:231-324): C = h^3 c s - beta q_i q_j U at the corner,
+  // X = (1 - rho W C)^-1 W C W, T = ((X - C) - beta q_i q_j U) s / L^3
+  const int ms = g.msol;
+  double Cr[MSX][MSX], Ci[MSX][MSX], Wm[MSX][MSX];
+  for (int i = 0; i < ms; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const int p = i * (i + 1) / 2 + j;
+      const cpx<T> c = lds[(p * LINES + l) * LS + lpos(z)];
+      const double q2 = g.q[i] * g.q[j];
+      const double re = (double)c.re * g.h3 * sgn - g.beta * q2 * U;
+      const double im = (double)c.im * g.h3 * sgn;
+      Cr[i][j] = Cr[j][i] = re;
+      Ci[i][j] = Ci[j][i] = im;
+      Wm[i][j] = Wm[j][i] = (i == j) ? 1.0 : (double)chi3[p * g.K + kk];
+    }
+  // WC, A = 1 - rho WC, B = WC W
+  double WCr[MSX][MSX], WCi[MSX][MSX], Ar[MSX][MSX], Ai[MSX][MSX], Br[MSX][MSX],
+      Bi[MSX][MSX];
+  for (int i = 0; i < ms; ++i)
+    for (int j = 0; j < ms; ++j) {
+      double ar = 0.0, ai = 0.0;
+      for (int q = 0; q < ms; ++q) { ar += Wm[i][q] * Cr[q][j]; ai += Wm[i][q] * Ci[q][j]; }
+      WCr[i][j] = ar;
+      WCi[i][j] = ai;
+      Ar[i][j] = (i == j ? 1.0 : 0.0) - g.rho * ar;
+      Ai[i][j] = -g.rho * ai;
+    }
+  for (int i = 0; i < ms; ++i)
+    for (int j = 0; j < ms; ++j) {
+      double br = 0.0, bi = 0.0;
+      for (int q = 0; q < ms; ++q) { br += WCr[i][q] * Wm[q][j]; bi += WCi[i][q] * Wm[q][j]; }
+      Br[i][j] = br;
+      Bi[i][j] = bi;
+    }
+  // Gaussian elimination with partial pivoting, A X = B
+  for (int col = 0; col < ms; ++col) {
+    int piv = col;
+    double best = Ar[col][col] * Ar[col][col] + Ai[col][col] * Ai[col][col];
+    for (int r = col + 1; r < ms; ++r) {
+      const double a2 = Ar[r][col] * Ar[r][col] + Ai[r][col] * Ai[r][col];
+      if (a2 > best) { best = a2; piv = r; }
+    }
+    if (piv != col)
+      for (int q = 0; q < ms; ++q) {
+        double tr = Ar[col][q], ti = Ai[col][q];
+        Ar[col][q] = Ar[piv][q]; Ai[col][q] = Ai[piv][q]; Ar[piv][q] = tr; Ai[piv][q] = ti;
+        tr = Br[col][q]; ti = Bi[col][q];
+        Br[col][q] = Br[piv][q]; Bi[col][q] = Bi[piv][q]; Br[piv][q] = tr; Bi[piv][q] = ti;
+      }
+    const double dr = Ar[col][col], di = Ai[col][col];
+    const double den = dr * dr + di * di;
+    const double ir = dr / den, ii = -di / den;  // 1 / A[col][col]
+    for (int r = col + 1; r < ms; ++r) {
+      const double fr = Ar[r][col] * ir - Ai[r][col] * ii;
+      const double fi = Ar[r][col] * ii + Ai[r][col] * ir;
+      for (int q = col; q < ms; ++q) {
+        Ar[r][q] -= fr * Ar[col][q] - fi * Ai[col][q];
+        Ai[r][q] -= fr * Ai[col][q] + fi * Ar[col][q];
+      }
+      for (int q = 0; q < ms; ++q) {
+        Br[r][q] -= fr * Br[col][q] - fi * Bi[col][q];
+        Bi[r][q] -= fr * Bi[col][q] + fi * Br[col][q];
+      }
+    }
+  }
+  for (int col = ms - 1; col >= 0; --col) {
+    const double dr = Ar[col][col], di = Ai[col][col];
+    const double den = dr * dr + di * di;
+    const double ir = dr / den, ii = -di / den;
+    for (int q = 0; q < ms; ++q) {
+      double xr = Br[col][q], xi = Bi[col][q];
+      for (int r = col + 1; r < ms; ++r) {
+        xr -= Ar[col][r] * Br[r][q] - Ai[col][r] * Bi[r][q];
+        xi -= Ar[col][r] * Bi[r][q] + Ai[col][r] * Br[r][q];
+      }
+      Br[col][q] = xr * ir - xi * ii;
+      Bi[col][q] = xr * ii + xi * ir;
+    }
+  }
+  for (int i = 0; i < ms; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const int p = i * (i + 1) / 2 + j;
+      const double q2 = g.q[i] * g.q[j];
+      const double tr = ((Br[i][j] - Cr[i][j]) - g.beta * q2 * U) * sgn * g.inv_L3;
+      const double ti = (Bi[i][j] - Ci[i][j]) * sgn * g.inv_L3;
+      lds[(p * LINES + l) * LS + lpos(z)] = cpx<T>{T(tr), T(ti)};
+    }
+}
+
 // --------------------------- 3. z c2c + chi contraction + tau + z c2c^-1
 // Grid: zorder 0: x = solute (concurrent workgroups share one chi3/tau3
 // tile), y = ceil(NH / LINES), z = ky row; zorder 1: x = kx tile, y = ky
@@ -471,93 +570,8 @@ zstar(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restrict__ 
           lds[(s * LINES + l) * LS + lpos(z)] = cpx<T>{T(U * sgn * g.inv_L3), T(0)};
         continue;
       }
-      // solvent-solvent OZ (iterate_t2, hnc3d.c:376-470; compute_t2_m,
-      // hnc3d.c:231-324): C = h^3 c s - beta q_i q_j U at the corner,
-      // X = (1 - rho W C)^-1 W C W, T = ((X - C) - beta q_i q_j U) s / L^3
-      const int ms = g.msol;
-      double Cr[3][3], Ci[3][3], Wm[3][3];
-      for (int i = 0; i < ms; ++i)
-        for (int j = 0; j <= i; ++j) {
-          const int p = i * (i + 1) / 2 + j;
-          const cpx<T> c = lds[(p * LINES + l) * LS + lpos(z)];
-          const double q2 = g.q[i] * g.q[j];
-          const double re = (double)c.re * g.h3 * sgn - g.beta * q2 * U;
-          const double im = (double)c.im * g.h3 * sgn;
-          Cr[i][j] = Cr[j][i] = re;
-          Ci[i][j] = Ci[j][i] = im;
-          Wm[i][j] = Wm[j][i] = (i == j) ? 1.0 : (double)chi3[p * g.K + kk];
-        }
-      // WC, A = 1 - rho WC, B = WC W
-      double WCr[3][3], WCi[3][3], Ar[3][3], Ai[3][3], Br[3][3], Bi[3][3];
-      for (int i = 0; i < ms; ++i)
-        for (int j = 0; j < ms; ++j) {
-          double ar = 0.0, ai = 0.0;
-          for (int q = 0; q < ms; ++q) { ar += Wm[i][q] * Cr[q][j]; ai += Wm[i][q] * Ci[q][j]; }
-          WCr[i][j] = ar;
-          WCi[i][j] = ai;
-          Ar[i][j] = (i == j ? 1.0 : 0.0) - g.rho * ar;
-          Ai[i][j] = -g.rho * ai;
-        }
-      for (int i = 0; i < ms; ++i)
-        for (int j = 0; j < ms; ++j) {
-          double br = 0.0, bi = 0.0;
-          for (int q = 0; q < ms; ++q) { br += WCr[i][q] * Wm[q][j]; bi += WCi[i][q] * Wm[q][j]; }
-          Br[i][j] = br;
-          Bi[i][j] = bi;
-        }
-      // Gaussian elimination with partial pivoting, A X = B (ms <= 3)
-      for (int col = 0; col < ms; ++col) {
-        int piv = col;
-        double best = Ar[col][col] * Ar[col][col] + Ai[col][col] * Ai[col][col];
-        for (int r = col + 1; r < ms; ++r) {
-          const double a2 = Ar[r][col] * Ar[r][col] + Ai[r][col] * Ai[r][col];
-          if (a2 > best) { best = a2; piv = r; }
-        }
-        if (piv != col)
-          for (int q = 0; q < ms; ++q) {
-            double tr = Ar[col][q], ti = Ai[col][q];
-            Ar[col][q] = Ar[piv][q]; Ai[col][q] = Ai[piv][q]; Ar[piv][q] = tr; Ai[piv][q] = ti;
-            tr = Br[col][q]; ti = Bi[col][q];
-            Br[col][q] = Br[piv][q]; Bi[col][q] = Bi[piv][q]; Br[piv][q] = tr; Bi[piv][q] = ti;
-          }
-        const double dr = Ar[col][col], di = Ai[col][col];
-        const double den = dr * dr + di * di;
-        const double ir = dr / den, ii = -di / den;  // 1 / A[col][col]
-        for (int r = col + 1; r < ms; ++r) {
-          const double fr = Ar[r][col] * ir - Ai[r][col] * ii;
-          const double fi = Ar[r][col] * ii + Ai[r][col] * ir;
-          for (int q = col; q < ms; ++q) {
-            Ar[r][q] -= fr * Ar[col][q] - fi * Ai[col][q];
-            Ai[r][q] -= fr * Ai[col][q] + fi * Ar[col][q];
-          }
-          for (int q = 0; q < ms; ++q) {
-            Br[r][q] -= fr * Br[col][q] - fi * Bi[col][q];
-            Bi[r][q] -= fr * Bi[col][q] + fi * Br[col][q];
-          }
-        }
-      }
-      for (int col = ms - 1; col >= 0; --col) {
-        const double dr = Ar[col][col], di = Ai[col][col];
-        const double den = dr * dr + di * di;
-        const double ir = dr / den, ii = -di / den;
-        for (int q = 0; q < ms; ++q) {
-          double xr = Br[col][q], xi = Bi[col][q];
-          for (int r = col + 1; r < ms; ++r) {
-            xr -= Ar[col][r] * Br[r][q] - Ai[col][r] * Bi[r][q];
-            xi -= Ar[col][r] * Bi[r][q] + Ai[col][r] * Br[r][q];
-          }
-          Br[col][q] = xr * ir - xi * ii;
-          Bi[col][q] = xr * ii + xi * ir;
-        }
-      }
-      for (int i = 0; i < ms; ++i)
-        for (int j = 0; j <= i; ++j) {
-          const int p = i * (i + 1) / 2 + j;
-          const double q2 = g.q[i] * g.q[j];
-          const double tr = ((Br[i][j] - Cr[i][j]) - g.beta * q2 * U) * sgn * g.inv_L3;
-          const double ti = (Bi[i][j] - Ci[i][j]) * sgn * g.inv_L3;
-          lds[(p * LINES + l) * LS + lpos(z)] = cpx<T>{T(tr), T(ti)};
-        }
+      if (g.msol <= 3) oz_pairs<T, 3, LINES, LS>(lds, g, chi3, kk, l, z, sgn, U);
+      else oz_pairs<T, MAX_SITES, LINES, LS>(lds, g, chi3, kk, l, z, sgn, U);
     }
   } else {
   const cpx<T>* Fs = static_cast<const cpx<T>*>(d.Fs);

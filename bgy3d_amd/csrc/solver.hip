@@ -1995,7 +1995,7 @@ std::mutex g_mutex;  // one solve per process at a time (shared streams)
 int solvent_run(const rism3d_problem* pd, int m, const rism3d_site* solvent,
                 const rism3d_options& opt, const double* t_init, double* g_out,
                 double* chi_out, double* t_out, rism3d_result* res) {
-  if (!pd || !solvent || !res || m < 1 || m > 3) return RISM3D_BAD_ARG;
+  if (!pd || !solvent || !res || m < 1 || m > MAX_SITES) return RISM3D_BAD_ARG;
   for (int d = 0; d < 3; ++d) {
     if (!supported_n(pd->N[d])) return RISM3D_UNSUPPORTED;
     if (!(pd->L[d] > 0)) return RISM3D_BAD_ARG;

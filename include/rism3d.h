@@ -345,7 +345,9 @@ int rism3d_jacobian(const rism3d_problem *pd,
 /* The 3D solvent-solvent solve, replacing hnc3d_solvent_solve
    (hnc3d.c:804-1073): site-site pair correlations on the 3D grid, centred
    at the box centre, from the OZ equation with the intramolecular
-   omega(k); m <= 3 solvent sites, cubic box, FP64, MDIIS or Picard.
+   omega(k); m <= 8 solvent sites (RISM3D_BAD_ARG beyond) whose
+   m(m+1)/2 pair spectra of two z lines fit 64 KB (RISM3D_UNSUPPORTED
+   otherwise: 6 sites up to N2 = 96), cubic box, FP64, MDIIS or Picard.
    Pairs are indexed p = i(i+1)/2 + j, j <= i.
    t_init: NULL (start from t = 0) or double[m(m+1)/2][N2][N1][N0];
    g_out:  NULL or double[m(m+1)/2][N2][N1][N0], g_ij = 1 + h_ij (the

@@ -124,14 +124,18 @@ def test_validation_before_device():
 
 
 def test_solvent_solve_validation():
-    """rism3d_solvent_solve: m <= 3 sites (the per-k OZ is solved by
-    elimination in registers), cubic boxes only (omega_ij(k) takes L[0] for
-    every axis, as bgy3d-pure.c:236-267), FP64 MDIIS or Picard."""
+    """rism3d_solvent_solve: m <= 8 sites (the per-k OZ is solved by
+    elimination per k point), the site-site pairs of a z line tile within
+    64 KB of LDS, cubic boxes only (omega_ij(k) takes L[0] for every axis,
+    as bgy3d-pure.c:236-267), FP64 MDIIS or Picard."""
     R = _R()
     base = {"L": 10, "N": 32}
     with pytest.raises(R.RismError) as e:
-        R.hnc3d_run_solvent("methanol", base)          # 6 sites
+        R.hnc3d_run_solvent("hexane", base)            # 20 sites
     assert e.value.status == R.BAD_ARG
+    with pytest.raises(R.RismError) as e:
+        R.hnc3d_run_solvent("methanol", dict(base, N=128))  # 21 pairs x 129 points x 2 lines
+    assert e.value.status == R.UNSUPPORTED
     with pytest.raises(R.RismError) as e:
         R.hnc3d_run_solvent("water", dict(base, precision="mixed"))
     assert e.value.status == R.UNSUPPORTED

@@ -66,6 +66,61 @@ def test_solvent_vs_oracle_32():
     assert dg < 1e-8 and dchi < 1e-8
 
 
+def test_methanol_solvent3d_vs_oracle_32():
+    """More than three solvent sites (the per-k OZ solve for m up to 8):
+    methanol, six sites and 21 pairs, at 32^3 against the oracle's solve
+    (tests/golden/methanol_solvent3d_32.json, make_methanol_solvent3d.py):
+    mu by the three functionals to 1e-8 and the charge, dipole and
+    diagonal second moments of 1 - g_ij of every pair to 1e-8; then the
+    same molecule with its sites listed O, OH first gives the same mu and
+    the permuted pair functions."""
+    print("case solvent3d methanol n32 vs oracle", flush=True)
+    from bgy3d_amd import rism3d as R
+    from bgy3d_amd.solutes import find_molecule
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "methanol_solvent3d_32.json")) as f:
+        gold = json.load(f)
+    st = gold["settings"]
+    rho, N, L = st["rho"], st["N"], st["L"]
+    meoh = find_molecule("methanol")
+    settings = {"N": N, "L": L, "beta": BETA, "rho": rho, "closure": "KH", "rmax": st["rmax"],
+                "nrad": st["nrad"], "max-iter": 3000, "norm-tol": 1e-11}
+
+    def run(mol):
+        pd = O.Problem.from_cli(L, N, BETA, rho, closure=C.KH, rmax=st["rmax"], nrad=st["nrad"])
+        one = R1.rism_vv(C.KH, st["nrad"], st["rmax"], BETA, rho, mol)
+        t0 = S3.Solvent3D(pd, mol).initial_guess(one["t"], st["rmax"])
+        return pd, R.hnc3d_run_solvent(mol, settings, t_init=t0)
+
+    pd, out = run(meoh)
+    print(f"  {out['status']} iters {out['iterations']} mu {out['free-energy-hnc']:.10f} "
+          f"{out['free-energy-kh']:.10f} {out['free-energy-gf']:.10f} oracle "
+          f"{gold['mu_hnc']:.10f} {gold['mu_kh']:.10f} {gold['mu_gf']:.10f}", flush=True)
+    assert out["status"] == "OK"
+    for k in ("hnc", "kh", "gf"):
+        assert abs(out["free-energy-" + k] - gold["mu_" + k]) < 1e-8, k
+    worst = 0.0
+    for p, gp in enumerate(gold["pairs"]):
+        q, d, Q = S3.moments(pd, 1.0 - out["GVV"][p])
+        got = [q] + list(d) + [Q[a][a] for a in range(3)]
+        want = [gp["q"]] + gp["d"] + gp["Qdiag"]
+        worst = max(worst, max(abs(x - y) for x, y in zip(got, want)))
+    print(f"  max |moment - oracle| {worst:.2e}", flush=True)
+    assert worst < 1e-8
+    perm = (4, 5, 0, 1, 2, 3)
+    _, outp = run(tuple(meoh[k] for k in perm))
+    assert outp["status"] == "OK"
+    for k in ("hnc", "kh", "gf"):
+        assert abs(outp["free-energy-" + k] - out["free-energy-" + k]) < 1e-8, k
+    dg = 0.0
+    for q_, (i, j) in enumerate((i, j) for i in range(6) for j in range(i + 1)):
+        oi, oj = perm[i], perm[j]
+        a, b = max(oi, oj), min(oi, oj)
+        dg = max(dg, float(np.max(np.abs(outp["GVV"][q_] - out["GVV"][a * (a + 1) // 2 + b]))))
+    print(f"  reordered sites: max |dg| {dg:.2e}", flush=True)
+    assert dg < 1e-8
+
+
 def test_solute_on_chi3d_vs_oracle_32():
     """The solute kernel on the oracle's own 3D chi (same input table):
     mu and g against oracle/rism3d.py Solute3D(chi3d=)."""
