@@ -129,11 +129,66 @@ __device__ __forceinline__ void stbf4(uint16_t* p, BF4 q) {
   x[1] = q.hi;
   *(G*)p = x;
 }
+// FP8 e4m3 (OCP: largest finite 448; the conversion does not saturate, so
+// values are clamped first) of four points scaled by s, packed in a word.
+__device__ __forceinline__ uint32_t to_fp8x4(const Vec16<float>& v, float s) {
+  auto c = [s](float x) { return fminf(fmaxf(x * s, -448.f), 448.f); };
+  int w = 0;
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(c(v.v[0]), c(v.v[1]), w, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(c(v.v[2]), c(v.v[3]), w, true);
+  return (uint32_t)w;
+}
+__device__ __forceinline__ Vec16<float> from_fp8x4(uint32_t w, float inv) {
+  Vec16<float> x;
+  x.v[0] = __builtin_amdgcn_cvt_f32_fp8((int)w, 0) * inv;
+  x.v[1] = __builtin_amdgcn_cvt_f32_fp8((int)w, 1) * inv;
+  x.v[2] = __builtin_amdgcn_cvt_f32_fp8((int)w, 2) * inv;
+  x.v[3] = __builtin_amdgcn_cvt_f32_fp8((int)w, 3) * inv;
+  return x;
+}
 __device__ __forceinline__ BF4 to_bf4(const Vec16<float>& v) {
   BF4 q;
   q.lo = bf16_bits(v.v[0]) | (bf16_bits(v.v[1]) << 16);
   q.hi = bf16_bits(v.v[2]) | (bf16_bits(v.v[3]) << 16);
   return q;
+}
+// The FP32 phase's residual history (Gram rows only): four points of entry
+// j at element idx as one packed word (BF16 pairs, or an FP8 word in lo),
+// unpacked to FP32, and the new r stored rounded (returned as rounded, so
+// that the Gram row is formed from the values the history keeps).
+__device__ __forceinline__ BF4 ld_hist(const SoluteDesc& d, const void* p, int64_t idx) {
+  if (d.r8) {
+    BF4 q;
+    q.lo = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(p) + idx);
+    q.hi = 0;
+    return q;
+  }
+  return ldbf4(static_cast<const uint16_t*>(p) + idx);
+}
+__device__ __forceinline__ Vec16<float> un_hist(const SoluteDesc& d, int j, BF4 q) {
+  if (d.r8) return from_fp8x4(q.lo, d.rhinv[j]);
+  Vec16<float> x;
+  x.v[0] = bf16_val(q.lo & 0xFFFFu);
+  x.v[1] = bf16_val(q.lo >> 16);
+  x.v[2] = bf16_val(q.hi & 0xFFFFu);
+  x.v[3] = bf16_val(q.hi >> 16);
+  return x;
+}
+__device__ __forceinline__ Vec16<float> st_hist(const SoluteDesc& d, int64_t idx,
+                                                const Vec16<float>& r) {
+  if (d.r8) {
+    const uint32_t w = to_fp8x4(r, d.rsc);
+    *reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(d.r) + idx) = w;
+    return from_fp8x4(w, 1.0f / d.rsc);
+  }
+  const BF4 q = to_bf4(r);
+  stbf4(static_cast<uint16_t*>(d.r) + idx, q);
+  Vec16<float> x;
+  x.v[0] = bf16_val(q.lo & 0xFFFFu);
+  x.v[1] = bf16_val(q.lo >> 16);
+  x.v[2] = bf16_val(q.hi & 0xFFFFu);
+  x.v[3] = bf16_val(q.hi >> 16);
+  return x;
 }
 
 // Index of the k point (kz, ky, kx) in a solute's form factor F(k) s(k):
@@ -904,7 +959,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   // the FFT instead of following it.  ITP vectors per thread.
   constexpr int ITP = (ROWS * N0) / (NT * VW);
   constexpr bool PREF = sizeof(T) == 4 && ITP * NT * VW == ROWS * N0 && ITP <= 2;
-  const bool pref = PREF && d.r16 != 0 && d.mode == 5 && vrows == ROWS;
+  const bool pref = PREF && (d.r16 != 0 || d.r8 != 0) && d.mode == 5 && vrows == ROWS;
   Vec16<T> ptv[PREF ? ITP : 1], pfb[PREF ? ITP : 1];
   BF4 prh[PREF ? ITP : 1][H];
   {
@@ -930,7 +985,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
           pfb[it] = ld16(static_cast<const T*>(d.fb) + e);
 #pragma unroll
           for (int j = 0; j < H; ++j)
-            if (j < d.ndot) prh[it][j] = ldbf4(static_cast<const uint16_t*>(d.rh[j]) + e);
+            if (j < d.ndot) prh[it][j] = ld_hist(d, d.rh[j], e);
         }
       }
     }
@@ -957,7 +1012,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
   const T* __restrict__ t = static_cast<const T*>(d.t) + off;
   T* __restrict__ r = static_cast<T*>(d.r) + off;  // (BF16 slots: addressed in e below)
   T* __restrict__ un = d.unew ? static_cast<T*>(d.unew) + off : nullptr;  // null: no u
-  const bool r16 = d.r16 != 0;  // FP32 phase: r history in BF16
+  const bool r16 = d.r16 != 0 || d.r8 != 0;  // FP32 phase: r history in BF16 or FP8
   const int nd = d.ndot;
   const T lam = T(d.lam);
   double mx = 0.0, ss = 0.0;
@@ -1010,14 +1065,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
     bool stored = false;
     if constexpr (sizeof(T) == 4) {
       if (r16) {  // store rounded, and form the Gram row from the rounded values
-        BF4 q;
-        q.lo = bf16_bits(rv.v[0]) | (bf16_bits(rv.v[1]) << 16);
-        q.hi = bf16_bits(rv.v[2]) | (bf16_bits(rv.v[3]) << 16);
-        stbf4(static_cast<uint16_t*>(d.r) + off + e, q);
-        rv.v[0] = bf16_val(q.lo & 0xFFFFu);
-        rv.v[1] = bf16_val(q.lo >> 16);
-        rv.v[2] = bf16_val(q.hi & 0xFFFFu);
-        rv.v[3] = bf16_val(q.hi >> 16);
+        rv = st_hist(d, off + e, rv);
         stored = true;
       }
     }
@@ -1038,14 +1086,9 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
         }
     }
   };
-  auto unbf = [](BF4 q) {
+  auto unbf = [&](int j, BF4 q) {
     Vec16<T> x;
-    if constexpr (sizeof(T) == 4) {
-      x.v[0] = bf16_val(q.lo & 0xFFFFu);
-      x.v[1] = bf16_val(q.lo >> 16);
-      x.v[2] = bf16_val(q.hi & 0xFFFFu);
-      x.v[3] = bf16_val(q.hi >> 16);
-    }
+    if constexpr (sizeof(T) == 4) x = un_hist(d, j, q);
     return x;
   };
   bool done = false;
@@ -1056,7 +1099,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
         Vec16<T> rh[H];
 #pragma unroll
         for (int j = 0; j < H; ++j)
-          if (j < nd) rh[j] = unbf(prh[it][j]);
+          if (j < nd) rh[j] = unbf(j, prh[it][j]);
         point(threadIdx.x * VW + it * NT * VW, ptv[it], pfb[it], rh);
       }
       done = true;
@@ -1073,7 +1116,7 @@ c2r_x_resid(const SoluteDesc* __restrict__ sd, GridArgs g,
         if (j < nd) {
           if constexpr (sizeof(T) == 4) {
             if (r16) {
-              rh[j] = unbf(ldbf4(static_cast<const uint16_t*>(d.rh[j]) + off + e));
+              rh[j] = unbf(j, ld_hist(d, d.rh[j], off + e));
               continue;
             }
           }
@@ -1267,7 +1310,7 @@ c2r_xy_resid(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __rest
   const T* __restrict__ t = static_cast<const T*>(d.t) + off;
   T* __restrict__ r = static_cast<T*>(d.r) + off;  // (BF16 slots: addressed in e below)
   T* __restrict__ un = d.unew ? static_cast<T*>(d.unew) + off : nullptr;  // null: no u
-  const bool r16 = d.r16 != 0;
+  const bool r16 = d.r16 != 0 || d.r8 != 0;  // FP32 phase: r history in BF16 or FP8
   const int nd = d.ndot;
   const int mode = d.mode;
   const T lam = T(d.lam);
@@ -1319,14 +1362,7 @@ c2r_xy_resid(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __rest
     bool stored = false;
     if constexpr (sizeof(T) == 4) {
       if (r16) {  // store rounded, and form the Gram row from the rounded values
-        BF4 b4;
-        b4.lo = bf16_bits(rv.v[0]) | (bf16_bits(rv.v[1]) << 16);
-        b4.hi = bf16_bits(rv.v[2]) | (bf16_bits(rv.v[3]) << 16);
-        stbf4(static_cast<uint16_t*>(d.r) + off + e, b4);
-        rv.v[0] = bf16_val(b4.lo & 0xFFFFu);
-        rv.v[1] = bf16_val(b4.lo >> 16);
-        rv.v[2] = bf16_val(b4.hi & 0xFFFFu);
-        rv.v[3] = bf16_val(b4.hi >> 16);
+        rv = st_hist(d, off + e, rv);
         stored = true;
       }
     }
@@ -1361,20 +1397,13 @@ c2r_xy_resid(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __rest
           if (mode == 5) f[q] = ld16(static_cast<const T*>(d.fb) + off + e);
 #pragma unroll
           for (int j = 0; j < H; ++j)
-            if (j < nd) rb[q][j] = ldbf4(static_cast<const uint16_t*>(d.rh[j]) + off + e);
+            if (j < nd) rb[q][j] = ld_hist(d, d.rh[j], off + e);
         }
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           const int e = e0 + q * NTP * VW;
           if (!WHOLE && e >= PLANE) continue;
-          point(e, tv[q], f[q], [&](int j) {
-            Vec16<T> x;
-            x.v[0] = bf16_val(rb[q][j].lo & 0xFFFFu);
-            x.v[1] = bf16_val(rb[q][j].lo >> 16);
-            x.v[2] = bf16_val(rb[q][j].hi & 0xFFFFu);
-            x.v[3] = bf16_val(rb[q][j].hi >> 16);
-            return x;
-          });
+          point(e, tv[q], f[q], [&](int j) { return un_hist(d, j, rb[q][j]); });
         }
       }
       done = true;
@@ -1497,7 +1526,7 @@ __global__ void __launch_bounds__(NT)
 rebase_prep(const double* __restrict__ tb, const double* __restrict__ bv,
             const double* __restrict__ f64, int py, int seed, float lam,
             float* __restrict__ xb, float* __restrict__ fb, float* __restrict__ delta,
-            float* __restrict__ u0, void* __restrict__ r0, int r16, int64_t n) {
+            float* __restrict__ u0, void* __restrict__ r0, int r16, float r8s, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * NT;
   for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < n; e += stride) {
     const double b = bv[e];
@@ -1506,7 +1535,10 @@ rebase_prep(const double* __restrict__ tb, const double* __restrict__ bv,
     fb[e] = f;
     if (seed) {
       delta[e] = 0.f;
-      if (r16)
+      if (r8s > 0.f)  // FP8 e4m3 of f * r8s (clamped: the conversion does not saturate)
+        static_cast<uint8_t*>(r0)[e] = (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(
+            fminf(fmaxf(f * r8s, -448.f), 448.f), 0.f, 0, false) & 0xFF);
+      else if (r16)
         static_cast<uint16_t*>(r0)[e] = (uint16_t)bf16_bits(f);
       else
         static_cast<float*>(r0)[e] = f;

@@ -50,6 +50,10 @@ struct SoluteDesc {
                              //   writes the new u over u[0] (unew) and its
                              //   difference from the old one to dnew
   void* dnew;                // u16: BF16 difference out
+  int r8;                    // FP32 (with r16): r and rh in FP8 e4m3 instead of
+                             //   BF16, each entry scaled into the e4m3 range
+  float rsc;                 // r8: the new r is stored as r * rsc
+  float rhinv[MAX_DIIS];     // r8: 1 / the scale of rh[j]
 };
 constexpr int NPART = 2 + MAX_DIIS;   // doubles per residual workgroup
 constexpr int INFO_NSUM = 22;         // grid sums of the info observables

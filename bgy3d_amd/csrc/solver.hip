@@ -562,6 +562,7 @@ struct SoluteState {
   float* dshift = nullptr; // mixed: the last delta folded into the base, by
                            // which entries of the previous base are moved
   float* ucur = nullptr;   // mixed, difference-form history: the newest u
+  std::vector<float> rsc;  // mixed, FP8 residual history: the scale of each slot
   int base = 0;            // mixed: rebases so far in this FP32 phase
   std::vector<int> sbase;  // mixed: base of each history slot
   double tol_lin = 0.0;    // adjoint: absolute tolerance (relative to |g|)
@@ -598,6 +599,9 @@ struct Problem {
   // FP32 phase (with r16): the u history in difference form, the newest u
   // in FP32 and the differences of consecutive entries in BF16
   int u16 = env_int("RISM3D_U_DIFF", 1);
+  // FP32 phase (with r16): the residual history in FP8 e4m3, each entry
+  // scaled by 240 / (max|r| of the previous pass) into the e4m3 range
+  int r8 = env_int("RISM3D_R_FP8", 1);
   int B;
   int64_t mP;
   hipStream_t s;
@@ -782,7 +786,7 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
       b0 += uin + (d.shift_w != 0.0 ? mP * sc : 0) + mP * sc + mP * sc + mK * 2 * sc;
       // W in, t in, r and u out, r_j in (+ F(t_b) in, defect correction);
       // r and r_j in BF16 with r16
-      const double sr = d.r16 ? 2.0 : sc;
+      const double sr = d.r8 ? 1.0 : (d.r16 ? 2.0 : sc);
       b4 += mK * 2 * sc + mP * sc + mP * sr + mP * sc + d.ndot * mP * sr +
             (d.mode == 5 ? mP * sc : 0) + (d.u16 ? mP * sc + mP * 2.0 : 0);
     }
@@ -1014,6 +1018,14 @@ void rebase(Problem& pb, const Config& cf, const std::vector<int>& mask, bool ad
 // delta just folded into t_b becomes the shift of the entries of the old
 // base (t = sum_j c_j u_j - (sum_{old j} c_j) shift), entries of the base
 // before are dropped, and the coefficients are refitted if any was.
+// FP8 residual history: the scale that puts max|r| = rinf at 240 (the
+// largest e4m3 value is 448; the next pass's residual may grow 1.8x before
+// it is clamped).
+float fp8_scale(double rinf) {
+  const double r = std::isfinite(rinf) && rinf > 1e-30 ? rinf : 1.0;
+  return (float)std::min(1e30, 240.0 / r);
+}
+
 // The FP32 phase's MDIIS step (RISM3D_STEP32, default the solve's step).
 double lam32(const Problem& pb, const Config& cf) { return pb.lam32 > 0 ? pb.lam32 : cf.lam; }
 
@@ -1035,9 +1047,15 @@ void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
                     : nullptr;
     if (pb.u16) U0 = s.ucur;
   }
+  float r8s = 0.f;  // FP8 history: r_0 = F(t_b) scaled by 240 / max|F(t_b)|
+  if (seed && pb.r8) {
+    s.rsc.assign(D, 1.f);
+    r8s = fp8_scale(s.resid);
+    s.rsc[0] = r8s;
+  }
   hipLaunchKernelGGL(rebase_prep, dim3(ch), dim3(NT), 0, pb.s, s.t64.p, s.bv64.p, s.r64.p,
                      pb.g.closure == RISM3D_CLOSURE_PY ? 1 : 0, seed ? 1 : 0, (float)lam32(pb, cf),
-                     s.bv32.p, s.fb32.p, s.dcur, U0, R0, pb.r16, mP);
+                     s.bv32.p, s.fb32.p, s.dcur, U0, R0, pb.r16, r8s, mP);
   s.best32 = INFINITY;
   s.stall32 = 0;
   if (seed) {
@@ -1179,7 +1197,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
       d.active = s.active && (s.phase32 != 0) == fp32;
       if (!d.active) continue;
       auto slotU = [&](int q) { return (void*)((char*)s.U.p + (size_t)q * mP * sz); };
-      const size_t szr = fp32 && pb.r16 ? 2 : sz;  // BF16 residual history
+      const size_t szr = fp32 && pb.r8 ? 1 : (fp32 && pb.r16 ? 2 : sz);  // FP8 / BF16 history
       auto slotR = [&](int q) { return (void*)((char*)s.R.p + (size_t)q * mP * szr); };
       d.t = fp32 ? (void*)s.dcur : (void*)s.t64.p;
       d.ncomb = (int)s.coef.size();
@@ -1217,6 +1235,12 @@ void iterate_batch(Problem& pb, const Config& cf) {
       }
       d.ndot = (int)s.dotted.size();
       for (int j = 0; j < d.ndot; ++j) d.rh[j] = slotR(s.dotted[j]);
+      if (fp32 && pb.r8) {
+        d.r8 = 1;
+        d.rsc = fp8_scale(s.resid);
+        s.rsc[s.slot_new] = d.rsc;
+        for (int j = 0; j < d.ndot; ++j) d.rhinv[j] = 1.0f / s.rsc[s.dotted[j]];
+      }
       d.r = slotR(s.slot_new);
       d.unew = u16 ? (void*)s.ucur : slotU(s.slot_new);
       d.dnew = u16 ? slotD(s.slot_new) : nullptr;
@@ -2210,6 +2234,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
   // iterates t itself to its floor, where the Gram rounding would show
   if (in.opt.precision != RISM3D_MIXED) pb.r16 = 0;
   if (!pb.r16) pb.u16 = 0;
+  if (!pb.r16) pb.r8 = 0;
   for (int d = 0; d < 3; ++d) { pb.N[d] = pd.N[d]; pb.L[d] = pd.L[d]; pb.h[d] = pd.L[d] / pd.N[d]; }
   GridArgs& g = pb.g;
   std::memset(&g, 0, sizeof g);
