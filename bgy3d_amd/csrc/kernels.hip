@@ -1584,34 +1584,95 @@ __device__ __forceinline__ double cl0(double r) {
 
 // beta v_i(r) for every solvent site (field0, bgy3d-solutes.c:344-377).
 // Grid: x = chunks of P, y = solute.  Writes FP64 and, if bv32 != NULL,
-// an FP32 copy for the mixed-precision bulk iterations.
+// an FP32 copy for the mixed-precision bulk iterations.  The solute sites
+// pass through LDS in chunks of FB_CHUNK (position, sigma, sqrt(epsilon),
+// charge / eps0); per point and site one r^2, 1 / r^2, r and the capped
+// erfc(G r) / r (within G r < 6.5), then per solvent site the Lorentz-Berthelot pair
+// (epsilon_ai = sqrt(eps_a) sqrt(eps_i), sigma_ai = (sigma_a + sigma_i) / 2)
+// with (sigma_ai / r)^6 from 1 / r^2: ljcap0(r / sigma_ai) without a
+// division or square root in the pair loop.
+constexpr int FB_CHUNK = 256;
+constexpr double FB_RC2 = (6.5 / G_COULOMB) * (6.5 / G_COULOMB);
+// MS: the solvent-site arrays' size (>= g.m), so that water's three fit
+// in few registers.
+template <int MS>
 __global__ void __launch_bounds__(NT)
 field_bv(const SitesDesc* __restrict__ sites, GridArgs g, const double* __restrict__ solvent_sse,
          double L0, double L1, double L2, double h0, double h1, double h2,
          double* const* __restrict__ bv64, float* const* __restrict__ bv32) {
+  __shared__ double st[FB_CHUNK][6];
   const SitesDesc sd = sites[blockIdx.y];
+  const int m = g.m;
+  double ssig[MS], sse[MS], sq[MS];
+  for (int i = 0; i < m; ++i) {
+    ssig[i] = solvent_sse[3 * i];
+    sse[i] = sqrt(solvent_sse[3 * i + 1]);
+    sq[i] = solvent_sse[3 * i + 2];
+  }
+  // ljcap0 below r = 0.2 sigma: a + b (x^2 - 0.04), x = r / sigma
+  const double R = 0.2, cap_a = lj0(R), cap_b = lj1(R) / (2 * R);
+  const int nch = (sd.n + FB_CHUNK - 1) / FB_CHUNK;
+  auto load = [&](int c) {
+    for (int k = threadIdx.x; k < FB_CHUNK; k += NT) {
+      const int a = c * FB_CHUNK + k;
+      if (a >= sd.n) break;
+      const double* s = sd.xyz_s_e_q + 6 * a;
+      st[k][0] = s[0];
+      st[k][1] = s[1];
+      st[k][2] = s[2];
+      st[k][3] = s[3];
+      st[k][4] = sqrt(s[4]);
+      st[k][5] = s[5] * EPSILON0INV;
+    }
+  };
+  if (nch == 1) {
+    load(0);
+    __syncthreads();
+  }
   const int64_t stride = (int64_t)gridDim.x * NT;
-  for (int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x; e < g.P; e += stride) {
+  // rounds of NT points, the same count in every thread (LDS reloads)
+  for (int64_t e0 = blockIdx.x * (int64_t)NT; e0 < g.P; e0 += stride) {
+    const int64_t e = e0 + threadIdx.x;
+    const bool ok = e < g.P;
     const int x = (int)(e % g.N0);
     const int y = (int)((e / g.N0) % g.N1);
     const int z = (int)(e / ((int64_t)g.N0 * g.N1));
     const double rx = x * h0 - L0 / 2, ry = y * h1 - L1 / 2, rz = z * h2 - L2 / 2;
-    double v[MAX_SITES];
-    for (int i = 0; i < g.m; ++i) v[i] = 0.0;
-    for (int a = 0; a < sd.n; ++a) {
-      const double* s = sd.xyz_s_e_q + 6 * a;
-      const double dx = rx - s[0], dy = ry - s[1], dz = rz - s[2];
-      const double rab = sqrt(dx * dx + dy * dy + dz * dz);
-      const double csc = G_COULOMB * cscap0(G_COULOMB * rab);
-      for (int i = 0; i < g.m; ++i) {
-        const double* sv = solvent_sse + 3 * i;  // sigma, epsilon, charge
-        const double eab = sqrt(sv[1] * s[4]);
-        const double sab = 0.5 * (sv[0] + s[3]);
-        const double qab = sv[2] * s[5] * EPSILON0INV;
-        v[i] += eab * ljcap0(rab / sab) + qab * csc;
+    double v[MS];
+    for (int i = 0; i < m; ++i) v[i] = 0.0;
+    for (int c = 0; c < nch; ++c) {
+      if (nch > 1) {
+        __syncthreads();
+        load(c);
+        __syncthreads();
+      }
+      const int na = min(FB_CHUNK, sd.n - c * FB_CHUNK);
+      if (!ok) continue;
+      for (int k = 0; k < na; ++k) {
+        const double dx = rx - st[k][0], dy = ry - st[k][1], dz = rz - st[k][2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double ir2 = 1.0 / r2;
+        // erfc(G r) / r vanishes beyond G r = 6.5 (erfc(6.5) = 3.8e-20: the
+        // term is below 2e-18 kcal/mol there, under the rounding of v)
+        const double csc =
+            r2 < FB_RC2 ? st[k][5] * G_COULOMB * cscap0(G_COULOMB * sqrt(r2)) : 0.0;
+        for (int i = 0; i < m; ++i) {
+          const double sab = 0.5 * (st[k][3] + ssig[i]);
+          const double sab2 = sab * sab;
+          const double s2 = sab2 * ir2;  // (sigma_ai / r)^2
+          double lj;
+          if (s2 <= 25.0) {
+            const double p6 = s2 * s2 * s2;
+            lj = 4 * p6 * (p6 - 1);
+          } else {
+            lj = cap_a + cap_b * (r2 / sab2 - R * R);
+          }
+          v[i] += st[k][4] * sse[i] * lj + sq[i] * csc;
+        }
       }
     }
-    for (int i = 0; i < g.m; ++i) {
+    if (!ok) continue;
+    for (int i = 0; i < m; ++i) {
       const double b = g.beta * v[i];
       bv64[blockIdx.y][i * g.P + e] = b;
       if (bv32) bv32[blockIdx.y][i * g.P + e] = (float)b;
@@ -1718,7 +1779,9 @@ mu_density(const SitesDesc* __restrict__ sites, GridArgs g, const double* const*
     for (int a = 0; a < sd.n; ++a) {
       const double* s = sd.xyz_s_e_q + 6 * a;
       const double dx = rx - s[0], dy = ry - s[1], dz = rz - s[2];
-      uc += s[5] * G_COULOMB * cl0(G_COULOMB * sqrt(dx * dx + dy * dy + dz * dz));
+      // erf(x) rounds to 1 in FP64 beyond x = 6 (erfc(6) = 2.2e-17)
+      const double xg = G_COULOMB * sqrt(dx * dx + dy * dy + dz * dz);
+      uc += s[5] * G_COULOMB * (xg >= 6.0 ? 1.0 / xg : cl0(xg));
     }
     uc *= EPSILON0INV;
     double v[5] = {0, 0, 0, 0, 0};

@@ -2413,7 +2413,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
     HIPCHK(hipMemcpyAsync(d_fs32.p, fs32p.data(), B * sizeof(void*), hipMemcpyHostToDevice, pb.s));
     const int cp = (int)std::min<int64_t>(4096, (g.P + NT - 1) / NT);
     const int ck = (int)std::min<int64_t>(4096, (g.K + NT - 1) / NT);
-    hipLaunchKernelGGL(field_bv, dim3(cp, B), dim3(NT), 0, pb.s, pb.d_sites.p, g, d_sse.p,
+    hipLaunchKernelGGL(g.m <= 3 ? field_bv<3> : field_bv<MAX_SITES>, dim3(cp, B), dim3(NT), 0, pb.s, pb.d_sites.p, g, d_sse.p,
                        pd.L[0], pd.L[1], pd.L[2], pb.h[0], pb.h[1], pb.h[2], d_bv64.p,
                        pb.mixed ? d_bv32.p : (float* const*)nullptr);
     hipLaunchKernelGGL(form_factor, dim3(ck, B), dim3(NT), 0, pb.s, pb.d_sites.p, g, d_fs64.p,
