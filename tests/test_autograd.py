@@ -46,10 +46,25 @@ def test_function_wiring(monkeypatch):
         assert float(f(p)) == pytest.approx(float(f(p.detach())))
 
 
-def test_no_renorm_rejected():
+def test_settings_reach_the_solve(monkeypatch):
+    """The solve's settings pass through unchanged, the Ng path ('no-renorm',
+    whose adjoint the library forms as on the tau path) and mixed precision
+    included, and the adjoint is asked for only when the parameters need a
+    gradient."""
+    seen = []
+
+    def fake(sites, solvent, settings, chi=None, adjoint=False):
+        seen.append((dict(settings), adjoint))
+        out = {"free-energy": 1.0, "status": "OK"}
+        if adjoint:
+            out["free-energy-dsite"] = np.zeros((len(sites), 6))
+        return out
+    monkeypatch.setattr(R, "hnc3d_run_solute", fake)
+    st = {"no-renorm": True, "precision": "mixed", "N": 32}
     p = torch.zeros((1, 6), dtype=torch.float64, requires_grad=True)
-    with pytest.raises(ValueError):
-        A.solute_mu(p, ["a"], "water", {"no-renorm": True})
+    A.solute_mu(p, ["a"], "water", st).backward()
+    A.solute_mu(p.detach(), ["a"], "water", st)
+    assert seen == [(st, True), (st, False)]
 
 
 def _gpu_present():
