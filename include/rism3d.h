@@ -99,7 +99,10 @@ enum rism3d_precision {
   RISM3D_FP64 = 0,          /* every kernel in FP64                     */
   RISM3D_MIXED = 1,         /* FP32 passes on a correction around an FP64
                                base (defect correction): FP64 residual at
-                               every rebase, FP64 polish; no FP32 floor  */
+                               every rebase, then polish_iters FP64 steps;
+                               no FP32 floor.  The FP32 MDIIS history is
+                               kept as the newest u in FP32, BF16
+                               differences of u and FP8 residuals        */
   RISM3D_FP32 = 2           /* FP32 passes only, around the starting
                                point: they stop at the FP32 floor (no
                                rebase, no polish); the FP64 residual there
