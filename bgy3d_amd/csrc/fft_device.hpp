@@ -247,9 +247,7 @@ __device__ __forceinline__ void fft_lines_il(cpx<T>* buf, int nl, const cpx<T>* 
 __host__ __device__ constexpr int rr_radix(int n, int r1, int s) {  // stage s >= 1
   int rem = n / r1;
   for (int i = 1;; ++i) {
-    int r;
-    if (rem <= r1) r = rem;
-    else r = rem % 8 == 0 && rem > 8 ? 8 : (rem % 4 == 0 && rem > 4 ? 4 : 2);
+    const int r = rem <= r1 ? rem : (rem % 8 == 0 && rem > 8 ? 8 : (rem % 4 == 0 && rem > 4 ? 4 : 2));
     if (i == s) return r;
     rem /= r;
     if (rem == 1) return 0;

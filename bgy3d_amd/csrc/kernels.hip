@@ -911,10 +911,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
       const int e = tid + p * NTZ;
-      if (PPT * NTZ == PTS || e < PTS) {
-        const int z = e / LINES, l = e - z * LINES;
-        lds[e] = c[s][p];
-      }
+      if (PPT * NTZ == PTS || e < PTS) lds[e] = c[s][p];
     }
     __syncthreads();
     fft_lines_il<T, N2, LINES, NTZ, true>(lds, nl, tw);
@@ -1476,6 +1473,288 @@ reduce_resid(const SoluteDesc* __restrict__ sd, int nparts, double* __restrict__
     for (int i = 1; i < NT / 64; ++i) a = (q == 0) ? fmax(a, red[q][i]) : a + red[q][i];
     out[blockIdx.x * NPART + q] = a;  // fmax drops NaNs; the sums keep them
   }
+}
+
+// v from lane `lane` (the same in every lane of the wavefront) to all lanes.
+__device__ __forceinline__ double lane_bcast(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// MDIIS step of the FP32 defect-correction phase on the device, after the
+// norm reduction of a pass (out): one wavefront per solute (grid x =
+// solute).  For every solute of the pass it does what the host loop does
+// between two passes: the stopping and rebase tests, the new entry into
+// the history and its Gram row, the restart test, the DIIS coefficients,
+// and the descriptor of the next pass (the combination t = u_newest -
+// sum_i (c_0 + .. + c_i) (u_{i+1} - u_i) over the BF16 differences, the
+// slot of the new entry, the FP8 scales).  A solute that wants a rebase or
+// failed is left inactive for the host.  The DIIS system is solved as
+// diis_coefficients solves it on the host, to the bit: LU with partial
+// pivoting (first largest pivot), one step of refinement against the
+// unfactored matrix, substitutions by columns in the host's order of
+// summation, no contraction into fused multiply-adds.
+__global__ void __launch_bounds__(64)
+mdiis_step(const double* __restrict__ out, MdiisDev* __restrict__ mst,
+           SoluteDesc* __restrict__ sd, MdiisStatus* __restrict__ status) {
+#pragma clang fp contract(off)
+  constexpr int NM = MAX_DIIS + 1;
+  constexpr int MD = MAX_DIIS;
+  __shared__ MdiisDev s;
+  __shared__ double o[NPART];
+  __shared__ double pcol[NM];  // |column| by row position, for the pivot scan
+  __shared__ int plane[NM];    // lane holding the row at each position
+  __shared__ int ctl[4];       // flag, -, -, LU ok
+  const int b = blockIdx.x, l = threadIdx.x;
+  SoluteDesc& d = sd[b];
+  {  // every load at once: the state, the pass's norms and dots
+    const int act = d.active;
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(mst + b);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(&s);
+    for (int i = l; i < (int)(sizeof(MdiisDev) / 8); i += 64) dst[i] = src[i];
+    if (l < NPART) o[l] = out[(size_t)b * NPART + l];
+    if (!act) return;
+  }
+  __syncthreads();
+  const double rinf = o[0], r2 = o[1];
+  const bool fin = __builtin_isfinite(rinf) && __builtin_isfinite(r2);
+  const int sn = s.slot_new, nd0 = s.ndot;
+  if (fin && l < nd0) {  // the new entry after the ones it was dotted with, its Gram row
+    const int q = s.dotted[l];
+    s.order[l] = q;
+    s.gram[sn * MD + q] = o[2 + l];
+    s.gram[q * MD + sn] = o[2 + l];
+  }
+  if (l == 0) {
+    s.iters++;
+    s.iters32++;
+    s.resid = rinf;
+    int flag = 0;
+    if (!fin) {
+      flag = 2;
+    } else {
+      if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; }
+      else if (rinf <= (s.only32 ? 1e-3 : 1e-2 * s.rb_rinf)) { s.stall32++; }
+      if ((!s.only32 && rinf <= 1e-3 * s.rb_rinf) || rinf <= s.tol_half || s.stall32 >= 12 ||
+          s.iters32 >= s.max_iter)
+        flag = 1;
+      s.sbase[sn] = s.base;
+      s.order[nd0] = sn;
+      s.n = nd0 + 1;
+      s.gram[sn * MD + sn] = r2;
+      // restart when the residual blows up against the best one, or stalls
+      if (r2 < s.best) { s.best = r2; s.since_best = 0; } else { s.since_best++; }
+      if (r2 > 100.0 * s.best || s.since_best > 3 * s.D) {
+        s.order[0] = sn;
+        s.n = 1;
+        s.best = r2;
+        s.since_best = 0;
+      }
+    }
+    ctl[0] = flag;
+    ctl[3] = 0;
+  }
+  __syncthreads();
+  const int flag = ctl[0];
+  if (flag != 2 && s.n > 1) {
+    // bordered system [G/sc -1; -1 0] [c; l] = [0; -1], sc = max diag G
+    const int nh = s.n, n = nh + 1;
+    double sc = 0.0;
+    for (int i = 0; i < nh; ++i) {
+      const double a = fabs(s.gram[s.order[i] * MD + s.order[i]]);
+      sc = sc < a ? a : sc;
+    }
+    if (sc > 0.0 && __builtin_isfinite(sc)) {
+      // lane l < n holds row l of the matrix, a (factored in place) and a0
+      // (as built); phys is the position the host's row swaps give that
+      // row, so position i holds the row of lane la[i] and perm[i] = la[i]
+      double a[NM], a0[NM];
+      const int oi = l < nh ? s.order[l] : 0;
+#pragma unroll
+      for (int j = 0; j < NM; ++j) {
+        double v = 0.0;
+        if (l < n && j < n)
+          v = (l < nh && j < nh) ? s.gram[oi * MD + s.order[j]] / sc
+                                 : (l == nh && j == nh ? 0.0 : -1.0);
+        a[j] = v;
+        a0[j] = v;
+      }
+      int phys = l < n ? l : NM + l;
+      bool ok = true;
+#pragma unroll
+      for (int col = 0; col < NM; ++col) {  // PA = LU, L unit lower
+        if (!ok || col >= n) continue;
+        // the host's scan over positions col..n-1: the first largest |A|
+        if (l < n) {
+          pcol[phys] = fabs(a[col]);
+          plane[phys] = l;
+        }
+        __syncthreads();
+        double v[NM];
+        int pl[NM];
+#pragma unroll
+        for (int q = col; q < NM; ++q) {
+          v[q] = pcol[q];
+          pl[q] = plane[q];
+        }
+        __syncthreads();
+        int piv = col, lp = pl[col];
+        double pmag = v[col];
+#pragma unroll
+        for (int q = col + 1; q < NM; ++q)
+          if (q < n && v[q] > pmag) {
+            pmag = v[q];
+            piv = q;
+            lp = pl[q];
+          }
+        const int lc = pl[col];
+        lp = __builtin_amdgcn_readfirstlane(lp);
+        if (!(pmag > 1e-300)) { ok = false; continue; }
+        if (l == lp) phys = col;  // positions col and piv swap rows
+        else if (l == lc) phys = piv;
+        const double inv = 1.0 / lane_bcast(a[col], lp);
+        double pr[NM];
+#pragma unroll
+        for (int k = col + 1; k < NM; ++k) pr[k] = lane_bcast(a[k], lp);
+        if (phys > col && l < n) {
+          const double f = a[col] * inv;
+          a[col] = f;
+#pragma unroll
+          for (int k = col + 1; k < NM; ++k)
+            if (k < n) a[k] -= f * pr[k];
+        }
+      }
+      if (ok) {
+        int la[NM];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+          const unsigned long long m = __ballot(phys == k);
+          la[k] = __builtin_amdgcn_readfirstlane(m ? __ffsll(m) - 1 : 0);
+        }
+        // x = LU^-1 b with b_l in lane l, x_l to lane l, twice: b, then
+        // the residual b - A x of one refinement step (x += A^-1 (b - A x)).
+        // Row sums run by columns, in the host's order of summation.
+        const double b0 = l == nh ? -1.0 : 0.0;
+        double rhs = b0, xv = 0.0, c = 0.0;
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+          double t = rhs, y = 0.0;
+#pragma unroll
+          for (int k = 0; k < NM; ++k) {  // forward, L unit lower
+            if (k < n) {
+              const double yk = lane_bcast(t, la[k]);
+              if (phys == k) y = t;
+              if (phys > k && l < n) t -= a[k] * yk;
+            }
+          }
+          t = y;
+          double x = 0.0;
+#pragma unroll
+          for (int k = NM - 1; k >= 0; --k) {  // backward, the last column first
+            if (k < n) {
+              const double xk = lane_bcast(t, la[k]) / lane_bcast(a[k], la[k]);
+              if (l == k) x = xk;
+              if (phys < k && l < n) t -= a[k] * xk;
+            }
+          }
+          if (it == 0) {
+            xv = x;
+            double r = b0;
+#pragma unroll
+            for (int k = 0; k < NM; ++k)
+              if (k < n) {
+                const double xk = lane_bcast(xv, k);
+                r -= a0[k] * xk;
+              }
+            rhs = r;
+          } else {
+            c = xv + x;
+          }
+        }
+        if (l < nh) s.coef[l] = c;
+        const bool bad = __ballot(l < nh && !__builtin_isfinite(c)) != 0;
+        if (l == 0) ctl[3] = bad ? 0 : 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (flag != 2 && (s.n == 1 || !ctl[3]) && l == 0) {  // one entry, or no usable solution: restart
+    s.order[0] = s.slot_new;
+    s.n = 1;
+    s.coef[0] = 1.0;
+  }
+  __syncthreads();
+  if (flag == 0) {
+    // the next pass: t = u_newest - sum_i (c_0 + .. + c_i) (u_{i+1} - u_i),
+    // the new r and u in a free slot, else in the oldest one (dropped)
+    const int n = s.n, D = s.D;
+    bool used = l >= D;
+    for (int i = 0; i < n; ++i) used |= s.order[i] == l;
+    const unsigned long long busy = __ballot(used);
+    const bool fresh = n < D;
+    const int q = fresh ? __ffsll(~busy) - 1 : s.order[0];
+    const int nd = fresh ? n : n - 1;
+    if (l + 1 < n) {
+      double cum = 0.0;
+      for (int i = 0; i <= l; ++i) cum += s.coef[i];
+      d.u[1 + l] = s.U + (size_t)s.order[l + 1] * s.mP;
+      d.coef[1 + l] = -cum;
+    }
+    const float rsc = fp8_scale(rinf);
+    if (l < nd) {
+      const int qq = s.order[fresh ? l : l + 1];
+      s.dotted[l] = qq;
+      d.rh[l] = s.R + (size_t)qq * s.mP;
+      d.rhinv[l] = 1.0f / s.rsc[qq];
+    }
+    if (l == 0) {
+      d.ncomb = n;
+      d.u[0] = s.ucur;
+      d.coef[0] = 1.0;
+      s.slot_new = q;
+      s.ndot = nd;
+      d.ndot = nd;
+      s.rsc[q] = rsc;
+      d.rsc = rsc;
+      d.r = s.R + (size_t)q * s.mP;
+      d.dnew = s.U + (size_t)q * s.mP;
+    }
+  } else if (l == 0) {
+    d.active = 0;
+  }
+  __syncthreads();
+  if (l == 0) {
+    // the iteration count last, released to the host that polls for it
+    MdiisStatus& st = status[b];
+    st.flag = flag;
+    st.iters32 = s.iters32;
+    st.hist = flag == 0 ? (s.n > s.ndot ? s.n : s.ndot) : 0;
+    st.resid = rinf;
+    st.r2 = r2;
+    __threadfence_system();
+    __hip_atomic_store(&st.iters, s.iters, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  {
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&s);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(mst + b);
+    for (int i = l; i < (int)(sizeof(MdiisDev) / 8); i += 64) dst[i] = src[i];
+  }
+}
+
+// Host-prepared rows into the device MDIIS state and descriptors, read
+// from pinned host memory: row r (grid x) is the solute index (8 bytes),
+// its SoluteDesc, then its MdiisDev.
+__global__ void __launch_bounds__(64)
+mdiis_load(const uint64_t* __restrict__ rows, SoluteDesc* __restrict__ sd,
+           MdiisDev* __restrict__ mst) {
+  constexpr int WS = sizeof(SoluteDesc) / 8, WM = sizeof(MdiisDev) / 8;
+  const uint64_t* row = rows + (size_t)blockIdx.x * (1 + WS + WM);
+  const int b = (int)row[0];
+  uint64_t* ds = reinterpret_cast<uint64_t*>(sd + b);
+  uint64_t* dm = reinterpret_cast<uint64_t*>(mst + b);
+  for (int i = threadIdx.x; i < WS; i += 64) ds[i] = row[1 + i];
+  for (int i = threadIdx.x; i < WM; i += 64) dm[i] = row[1 + WS + i];
 }
 
 // FP32 <-> FP64 copies of the iterate at a precision hand-over.

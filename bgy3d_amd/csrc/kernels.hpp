@@ -55,7 +55,60 @@ struct SoluteDesc {
   float rsc;                 // r8: the new r is stored as r * rsc
   float rhinv[MAX_DIIS];     // r8: 1 / the scale of rh[j]
 };
+static_assert(sizeof(SoluteDesc) % 8 == 0, "SoluteDesc is copied in 8-byte words");
 constexpr int NPART = 2 + MAX_DIIS;   // doubles per residual workgroup
+
+// MDIIS state of one solute in the FP32 defect-correction phase with the
+// difference-form u history and the FP8 residual history, kept on the
+// device between passes (mdiis_step): the history bookkeeping the host loop
+// would otherwise do between two residual passes.  The host takes a
+// solute's state back when it asks for a rebase or fails.  gram is indexed
+// by history slot, [slot][slot] with row stride MAX_DIIS.
+struct MdiisDev {
+  // fixed while the solute stays on the device
+  float* ucur;            // the newest u (FP32)
+  uint16_t* U;            // BF16 difference slots, mP each
+  uint8_t* R;             // FP8 residual slots, mP each
+  int64_t mP;
+  double rb_rinf;         // max|F(t_b)| of the current base
+  double tol_half;        // half the FP64-checked tolerance of the phase
+  int D;                  // history depth of the phase
+  int only32;             // FP32-only precision: no rebase on the 1e3 drop
+  int max_iter;
+  int base;               // rebases so far in this phase
+  // state
+  int n;                  // entries in order (oldest first)
+  int slot_new;           // slot receiving the pending r and u
+  int ndot;               // entries the pending r is dotted with (dotted)
+  int since_best, stall32, iters, iters32;
+  int pad_;
+  int order[MAX_DIIS];
+  int dotted[MAX_DIIS];
+  int sbase[MAX_DIIS];    // base of each slot's entry
+  float rsc[MAX_DIIS];    // FP8 scale of each slot's residual
+  double coef[MAX_DIIS];  // t = sum coef_j u_{order_j} for the next pass
+  double resid, best, best32;
+  double gram[MAX_DIIS * MAX_DIIS];
+};
+static_assert(sizeof(MdiisDev) % 8 == 0, "MdiisDev is copied in 8-byte words");
+
+// What mdiis_step reports to the host per solute and pass (pinned memory).
+struct MdiisStatus {
+  int flag;               // 0 next pass prepared, 1 rebase wanted, 2 not
+                          // finite, 3 not in this pass
+  int iters, iters32;
+  int hist;               // flag 0: history slots of the next pass, max(ncomb, ndot)
+  double resid, r2;       // max|F|, ||F||^2 of the pass
+};
+
+// FP8 residual history: the scale that puts max|r| = rinf at 240 (the
+// largest e4m3 value is 448; the next pass's residual may grow 1.8x before
+// it is clamped).
+__host__ __device__ inline float fp8_scale(double rinf) {
+  const double r = __builtin_isfinite(rinf) && rinf > 1e-30 ? rinf : 1.0;
+  const double v = 240.0 / r;
+  return (float)(v < 1e30 ? v : 1e30);
+}
 constexpr int INFO_NSUM = 22;         // grid sums of the info observables
 
 // Grid and physics constants shared by a batch.

@@ -477,8 +477,13 @@ void make_twiddles(Twiddles& tw, const int N[3], hipStream_t s) {
 // in FP64 by LU with partial pivoting and one step of iterative refinement
 // against the unfactored matrix, on the stack: it runs for every solute of
 // a batch between two residual passes, while the device waits.  Returns
-// false if the system is singular or the result not finite.
+// false if the system is singular or the result not finite.  mdiis_step
+// solves the same system on the device with the same operations in the
+// same order (sums of the back substitution from the last column down), and
+// neither contracts products into fused multiply-adds, so both give the
+// same coefficients to the bit.
 bool diis_coefficients(const std::vector<double>& G, int nh, std::vector<double>& c) {
+#pragma clang fp contract(off)
   constexpr int NM = MAX_DIIS + 1;
   const int n = nh + 1;
   if (nh < 1 || nh > MAX_DIIS) return false;
@@ -520,7 +525,7 @@ bool diis_coefficients(const std::vector<double>& G, int nh, std::vector<double>
     }
     for (int i = n - 1; i >= 0; --i) {
       double t = y[i];
-      for (int k = i + 1; k < n; ++k) t -= A[i][k] * out[k];
+      for (int k = n - 1; k > i; --k) t -= A[i][k] * out[k];
       out[i] = t / A[i][i];
     }
   };
@@ -624,11 +629,12 @@ struct Problem {
   // before each replay).  Re-captured if the grid arguments change.
   // RISM3D_GRAPHS=0 launches the kernels one by one instead.
   struct Graphs {
-    hipGraph_t graph[2][3] = {};  // [FP32][history width 8 / 10 / 16]
-    hipGraphExec_t exec[2][3] = {};
-    GridArgs key[2][3];
+    // [FP64 / FP32 / FP32 with the device MDIIS step][history width 8 / 10 / 16]
+    hipGraph_t graph[3][3] = {};
+    hipGraphExec_t exec[3][3] = {};
+    GridArgs key[3][3];
     ~Graphs() {
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) {
           if (exec[a][b]) (void)hipGraphExecDestroy(exec[a][b]);
           if (graph[a][b]) (void)hipGraphDestroy(graph[a][b]);
@@ -636,6 +642,20 @@ struct Problem {
     }
   } graphs;
   int use_graphs = env_int("RISM3D_GRAPHS", 1);
+  // The FP32 phase's MDIIS step on the device (mdiis_step, with the
+  // difference-form u and FP8 r histories): its own descriptors, which the
+  // step rewrites, the solutes' history state and the pinned status array
+  // the host reads after each pass.  RISM3D_DEVICE_MDIIS=0 keeps the step
+  // on the host.
+  int dev_mdiis = env_int("RISM3D_DEVICE_MDIIS", 1);
+  DevBuf<SoluteDesc> d_sd32;
+  DevBuf<MdiisDev> d_mdiis;
+  struct PinnedStatus {
+    MdiisStatus* p = nullptr;
+    ~PinnedStatus() {
+      if (p) (void)hipHostFree(p);
+    }
+  } h_status;
   // launch configuration of the residual chain per working precision
   // ([0] FP64, [1] FP32): defaults from the environment, or the autotuner's
   // choice (tune)
@@ -651,17 +671,17 @@ struct Problem {
 };
 
 // Descriptors to the device through pinned staging (stream-ordered).
-void upload_sd(Problem& pb, const std::vector<SoluteDesc>& sd) {
+void upload_sd(Problem& pb, const std::vector<SoluteDesc>& sd, SoluteDesc* dst) {
   const size_t bytes = sd.size() * sizeof(SoluteDesc);
   if (pb.noalloc) {  // from pageable memory, complete before sd can change
-    HIPCHK(hipMemcpyAsync(pb.d_sd.p, sd.data(), bytes, hipMemcpyHostToDevice, pb.s));
+    HIPCHK(hipMemcpyAsync(dst, sd.data(), bytes, hipMemcpyHostToDevice, pb.s));
     HIPCHK(hipStreamSynchronize(pb.s));
     return;
   }
   int k = 0;
   void* h = pb.up.slot(bytes, &k);
   std::memcpy(h, sd.data(), bytes);
-  HIPCHK(hipMemcpyAsync(pb.d_sd.p, h, bytes, hipMemcpyHostToDevice, pb.s));
+  HIPCHK(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, pb.s));
   pb.up.mark(k, pb.s);
 }
 // pb.d_out to out[], synchronising the stream.
@@ -704,19 +724,33 @@ int hist_of(const std::vector<SoluteDesc>& sd) {
   return h;
 }
 
-void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
+// One residual pass over the batch.  step 0: the host's descriptors sd.
+// step 1 (FP32): the descriptors already in pb.d_sd32, with hist2 history
+// slots at most; the device MDIIS step (mdiis_step) follows the pass and
+// writes the next pass's descriptors there.
+void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd, int step = 0, int hist2 = 0) {
   GridArgs g = pb.g;
   const Problem::Knobs& kn = pb.knob[fp32 ? 1 : 0];
   g.zorder = std::min(3, std::max(0, kn.zorder));
   g.zwide = kn.zwide;
   g.nyq = kn.nyq;
-  upload_sd(pb, sd);
-  const dim3 gx(resid_grid_x(g), g.m, pb.B);
+  SoluteDesc* dsd = step ? pb.d_sd32.p : pb.d_sd.p;
   Profile& pf = g_prof;
+  std::vector<SoluteDesc> sdp;  // profiling, step 2: the descriptors on the device
+  if (!step) {
+    upload_sd(pb, sd, dsd);
+  } else if (pf.on) {
+    sdp.resize(sd.size());
+    HIPCHK(hipMemcpyAsync(sdp.data(), dsd, sdp.size() * sizeof(SoluteDesc), hipMemcpyDeviceToHost,
+                          pb.s));
+    HIPCHK(hipStreamSynchronize(pb.s));
+  }
+  const std::vector<SoluteDesc>& sdb = sdp.empty() ? sd : sdp;
+  const dim3 gx(resid_grid_x(g), g.m, pb.B);
   auto mark = [&](int k, int which) {
     if (pf.on) HIPCHK(hipEventRecord(pf.ev[k][which], pb.s));
   };
-  const int hist = hist_of(sd);
+  const int hist = step ? hist2 : hist_of(sd);
   const bool planes = use_planes(g, fp32, kn.planes);
   if (planes) {
     // y-major spectra, the Nyquist column in the z pass's launch of its
@@ -729,33 +763,36 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
   auto chain = [&]() {
     if (planes) {
       using T = float;
-      mark(0, 0); launch_planes<T>(false, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, pb.tw.f[2].p, hist); mark(0, 1);
-      mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
-      mark(4, 0); launch_planes<T>(true, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, pb.tw.f[2].p, hist); mark(4, 1);
+      mark(0, 0); launch_planes<T>(false, pb.B, pb.s, dsd, g, pb.tw.f[1].p, pb.tw.f[0].p, pb.tw.f[2].p, hist); mark(0, 1);
+      mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, dsd, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
+      mark(4, 0); launch_planes<T>(true, pb.B, pb.s, dsd, g, pb.tw.f[1].p, pb.tw.f[0].p, pb.tw.f[2].p, hist); mark(4, 1);
     } else if (fp32) {
       using T = float;
       constexpr int LN = y_lines<T>();
       const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
-      mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, hist); mark(0, 1);
-      mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(1, 1);
-      mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
-      mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.f[2].p); mark(3, 1);
-      mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.f[1].p, pb.tw.f[0].p, hist); mark(4, 1);
+      mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, dsd, g, pb.tw.f[1].p, pb.tw.f[0].p, hist); mark(0, 1);
+      mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, dsd, g, pb.tw.f[2].p); mark(1, 1);
+      mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, dsd, g, pb.tw.f[3].p, pb.chi3f.p, pb.tau3f.p); mark(2, 1);
+      mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, dsd, g, pb.tw.f[2].p); mark(3, 1);
+      mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, dsd, g, pb.tw.f[1].p, pb.tw.f[0].p, hist); mark(4, 1);
     } else {
       using T = double;
       constexpr int LN = y_lines<T>();
       const dim3 gy((g.NX + LN - 1) / LN, g.m * g.N2, pb.B);
-      mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist); mark(0, 1);
-      mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(1, 1);
-      mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, pb.d_sd.p, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p); mark(2, 1);
-      mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, pb.d_sd.p, g, pb.tw.d[2].p); mark(3, 1);
-      mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, pb.d_sd.p, g, pb.tw.d[1].p, pb.tw.d[0].p, hist); mark(4, 1);
+      mark(0, 0); launch_closure_r2c<T>(g.N0, gx, pb.s, dsd, g, pb.tw.d[1].p, pb.tw.d[0].p, hist); mark(0, 1);
+      mark(1, 0); launch_fft_y<T, false>(g.N1, gy, pb.s, dsd, g, pb.tw.d[2].p); mark(1, 1);
+      mark(2, 0); launch_zstar<T>(g.N2, g.NX, g.N1, pb.B, pb.s, dsd, g, pb.tw.d[3].p, pb.chi3.p, pb.tau3.p); mark(2, 1);
+      mark(3, 0); launch_fft_y<T, true>(g.N1, gy, pb.s, dsd, g, pb.tw.d[2].p); mark(3, 1);
+      mark(4, 0); launch_c2r_resid<T>(g.N0, gx, pb.s, dsd, g, pb.tw.d[1].p, pb.tw.d[0].p, hist); mark(4, 1);
     }
-    hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, pb.d_sd.p, nparts, pb.d_out.p);
+    hipLaunchKernelGGL(reduce_resid, dim3(pb.B), dim3(NT), 0, pb.s, dsd, nparts, pb.d_out.p);
+    if (step)
+      hipLaunchKernelGGL(mdiis_step, dim3(pb.B), dim3(64), 0, pb.s, pb.d_out.p, pb.d_mdiis.p, dsd,
+                         pb.h_status.p);
   };
   if (pb.use_graphs && !pf.on) {
     auto& G = pb.graphs;
-    const int a = fp32 ? 1 : 0, b = hist <= 8 ? 0 : (hist <= 10 ? 1 : 2);
+    const int a = step ? 2 : (fp32 ? 1 : 0), b = hist <= 8 ? 0 : (hist <= 10 ? 1 : 2);
     if (!G.exec[a][b] || std::memcmp(&G.key[a][b], &g, sizeof g) != 0) {
       if (G.exec[a][b]) HIPCHK(hipGraphExecDestroy(G.exec[a][b]));
       if (G.graph[a][b]) HIPCHK(hipGraphDestroy(G.graph[a][b]));
@@ -777,7 +814,7 @@ void evaluate(Problem& pb, bool fp32, std::vector<SoluteDesc>& sd) {
     const double mP = (double)g.m * g.P, mK = (double)g.m * Kv;
     double b0 = 0, b4 = 0;
     int na = 0;
-    for (auto& d : sd) {
+    for (auto& d : sdb) {
       if (!d.active) continue;
       ++na;
       // u_j in (+ the shift of the previous base's entries), t out (or t
@@ -1018,13 +1055,6 @@ void rebase(Problem& pb, const Config& cf, const std::vector<int>& mask, bool ad
 // delta just folded into t_b becomes the shift of the entries of the old
 // base (t = sum_j c_j u_j - (sum_{old j} c_j) shift), entries of the base
 // before are dropped, and the coefficients are refitted if any was.
-// FP8 residual history: the scale that puts max|r| = rinf at 240 (the
-// largest e4m3 value is 448; the next pass's residual may grow 1.8x before
-// it is clamped).
-float fp8_scale(double rinf) {
-  const double r = std::isfinite(rinf) && rinf > 1e-30 ? rinf : 1.0;
-  return (float)std::min(1e30, 240.0 / r);
-}
 
 // The FP32 phase's MDIIS step (RISM3D_STEP32, default the solve's step).
 double lam32(const Problem& pb, const Config& cf) { return pb.lam32 > 0 ? pb.lam32 : cf.lam; }
@@ -1178,6 +1208,113 @@ void iterate_batch(Problem& pb, const Config& cf) {
     rebase(pb, cf, act, false, rinf, r2);
     after_rebase(act, rinf, r2, true);
   }
+  // the FP32 phase's MDIIS step on the device (mdiis_step): the history in
+  // difference form with FP8 residuals, the library's own allocations
+  const bool dev_ok = pb.dev_mdiis && pb.u16 && pb.r8 && pb.r16 && !pb.noalloc &&
+                      cf.precision != RISM3D_FP64;
+  bool dev_on = false;          // the FP32 solutes' state is on the device
+  std::vector<int> dev_dirty;   // solutes whose host state goes up before the next pass
+  std::vector<int> next_hist(B, 0);  // history slots of each solute's next pass
+  std::vector<MdiisDev> hm(B);
+  // host state of solute b (after make_desc) to its device form, and back
+  auto pack = [&](int b, int Dp) {
+    const auto& s = pb.st[b];
+    MdiisDev& m = hm[b];
+    std::memset(&m, 0, sizeof m);
+    m.ucur = s.ucur;
+    m.U = static_cast<uint16_t*>(static_cast<void*>(s.U.p));
+    m.R = static_cast<uint8_t*>(static_cast<void*>(s.R.p));
+    m.mP = mP;
+    m.rb_rinf = s.rb_rinf;
+    m.tol_half = 0.5 * s.tol_mixed;
+    m.D = Dp;
+    m.only32 = cf.precision == RISM3D_FP32;
+    m.max_iter = cf.max_iter;
+    m.base = s.base;
+    m.n = (int)s.order.size();
+    m.slot_new = s.slot_new;
+    m.ndot = (int)s.dotted.size();
+    m.since_best = s.since_best;
+    m.stall32 = s.stall32;
+    m.iters = s.iters;
+    m.iters32 = s.iters32;
+    for (int i = 0; i < m.n; ++i) {
+      m.order[i] = s.order[i];
+      m.coef[i] = s.coef[i];
+    }
+    for (int i = 0; i < m.ndot; ++i) m.dotted[i] = s.dotted[i];
+    for (int q = 0; q < D; ++q) {
+      m.sbase[q] = s.sbase[q];
+      m.rsc[q] = s.rsc[q];
+      for (int j = 0; j < D; ++j) m.gram[q * MAX_DIIS + j] = s.gram[q * D + j];
+    }
+    m.resid = s.resid;
+    m.best = s.best;
+    m.best32 = s.best32;
+  };
+  auto unpack = [&](int b, const MdiisDev& m) {
+    auto& s = pb.st[b];
+    s.order.assign(m.order, m.order + m.n);
+    s.coef.assign(m.coef, m.coef + m.n);
+    s.dotted.assign(m.dotted, m.dotted + m.ndot);
+    s.slot_new = m.slot_new;
+    for (int q = 0; q < D; ++q) {
+      s.sbase[q] = m.sbase[q];
+      s.rsc[q] = m.rsc[q];
+      for (int j = 0; j < D; ++j) s.gram[q * D + j] = m.gram[q * MAX_DIIS + j];
+    }
+    s.since_best = m.since_best;
+    s.stall32 = m.stall32;
+    s.iters = m.iters;
+    s.iters32 = m.iters32;
+    s.resid = m.resid;
+    s.best = m.best;
+    s.best32 = m.best32;
+  };
+  // the host's descriptors and device state of the given solutes into
+  // pb.d_sd32 and pb.d_mdiis: one launch of mdiis_load over pinned rows
+  auto load_rows = [&](const std::vector<int>& rows) {
+    constexpr size_t WS = sizeof(SoluteDesc) / 8, WM = sizeof(MdiisDev) / 8, W = 1 + WS + WM;
+    int k = 0;
+    uint64_t* h = static_cast<uint64_t*>(pb.up.slot(rows.size() * W * 8, &k));
+    for (size_t i = 0; i < rows.size(); ++i) {
+      const int b = rows[i];
+      h[i * W] = (uint64_t)b;
+      std::memcpy(h + i * W + 1, &sd[b], sizeof(SoluteDesc));
+      std::memcpy(h + i * W + 1 + WS, &hm[b], sizeof(MdiisDev));
+    }
+    hipLaunchKernelGGL(mdiis_load, dim3((unsigned)rows.size()), dim3(64), 0, pb.s, h,
+                       pb.d_sd32.p, pb.d_mdiis.p);
+    pb.up.mark(k, pb.s);
+  };
+  // The end of a pass with the device step: every solute of the pass has
+  // released its status with the next iteration count.  The host polls the
+  // pinned statuses instead of waiting for the stream (which wakes it
+  // later); a stream that ends or fails without them is an error.
+  auto wait_status = [&]() {
+    std::vector<int> expect(B, 0);
+    for (int b = 0; b < B; ++b)
+      if (pb.st[b].active && pb.st[b].phase32) expect[b] = pb.st[b].iters + 1;
+    auto arrived = [&]() {
+      for (int b = 0; b < B; ++b)
+        if (expect[b] && __atomic_load_n(&pb.h_status.p[b].iters, __ATOMIC_ACQUIRE) != expect[b])
+          return false;
+      return true;
+    };
+    for (long spin = 1;; ++spin) {
+      if (arrived()) break;
+      if (spin % 64 == 0) {
+        const hipError_t e = hipStreamQuery(pb.s);
+        if (e == hipSuccess) {
+          if (arrived()) break;
+          std::fprintf(stderr, "rism3d: mdiis_step: the pass ended without its statuses\n");
+          throw HipFailure{hipErrorUnknown};
+        }
+        if (e != hipErrorNotReady) HIPCHK(e);
+      }
+    }
+    HIPCHK(hipGetLastError());
+  };
   for (;;) {
     bool any = false, any32 = false;
     for (int b = 0; b < B; ++b) {
@@ -1190,12 +1327,13 @@ void iterate_batch(Problem& pb, const Config& cf) {
     const bool fp32 = any32;
     const size_t sz = fp32 ? 4 : 8;
     const int Dp = fp32 && pb.D32 > 0 ? std::min(D, pb.D32) : D;  // this phase's depth
-    for (int b = 0; b < B; ++b) {
+    // the descriptor of solute b's next pass from its host state
+    auto make_desc = [&](int b) {
       auto& s = pb.st[b];
       SoluteDesc& d = sd[b];
       std::memset(&d, 0, sizeof d);
       d.active = s.active && (s.phase32 != 0) == fp32;
-      if (!d.active) continue;
+      if (!d.active) return;
       auto slotU = [&](int q) { return (void*)((char*)s.U.p + (size_t)q * mP * sz); };
       const size_t szr = fp32 && pb.r8 ? 1 : (fp32 && pb.r16 ? 2 : sz);  // FP8 / BF16 history
       auto slotR = [&](int q) { return (void*)((char*)s.R.p + (size_t)q * mP * szr); };
@@ -1257,7 +1395,79 @@ void iterate_batch(Problem& pb, const Config& cf) {
       d.gv = (fp32 && cf.mode == 1) ? s.r64.p : s.gmu.p;
       d.fb = s.fb32.p;
       d.r16 = fp32 && pb.r16;
+    };
+    if (fp32 && dev_ok) {
+      // The MDIIS step on the device: the solutes' history state goes up
+      // once; after that the host takes back only the solutes that ask for
+      // a rebase or fail, and sends them up again after it.
+      if (!dev_on) {
+        if (!pb.d_mdiis.p) {
+          pb.d_sd32 = DevBuf<SoluteDesc>(B);
+          pb.d_mdiis = DevBuf<MdiisDev>(B);
+          HIPCHK(hipHostMalloc((void**)&pb.h_status.p, B * sizeof(MdiisStatus),
+                               hipHostMallocCoherent));
+        }
+        std::memset(pb.h_status.p, 0, B * sizeof(MdiisStatus));
+        dev_dirty.clear();
+        for (int b = 0; b < B; ++b) dev_dirty.push_back(b);
+        dev_on = true;
+      }
+      if (!dev_dirty.empty()) {
+        for (int b : dev_dirty) {
+          make_desc(b);
+          if (sd[b].active) pack(b, Dp);
+          next_hist[b] = std::max(sd[b].ncomb, sd[b].ndot);
+        }
+        load_rows(dev_dirty);
+        dev_dirty.clear();
+      }
+      int hist = 0;
+      for (int b = 0; b < B; ++b)
+        if (pb.st[b].active && pb.st[b].phase32) hist = std::max(hist, next_hist[b]);
+      evaluate(pb, true, sd, 1, hist);
+      wait_status();
+      std::vector<int> rebase_mask(B, 0), back;
+      bool any_rebase = false;
+      for (int b = 0; b < B; ++b) {
+        auto& s = pb.st[b];
+        if (!(s.active && s.phase32)) continue;
+        const MdiisStatus& q = pb.h_status.p[b];
+        s.iters = q.iters;
+        s.iters32 = q.iters32;
+        s.resid = q.resid;
+        next_hist[b] = q.hist;
+        if (cf.verbose)
+          std::fprintf(stderr, "rism3d: solute %d iter %d fp32 |F|inf = %.3e |F|2 = %.3e\n", b,
+                       s.iters, q.resid, std::sqrt(q.r2));
+        if (q.flag == 2) {
+          s.status = RISM3D_NAN;
+          s.active = 0;
+          back.push_back(b);
+        } else if (q.flag == 1) {
+          rebase_mask[b] = 1;
+          any_rebase = true;
+          back.push_back(b);
+        }
+      }
+      if (!back.empty()) {  // their state back to the host
+        const size_t bytes = (size_t)B * sizeof(MdiisDev);
+        int k = 0;
+        void* h = pb.down.slot(bytes, &k);
+        HIPCHK(hipMemcpyAsync(h, pb.d_mdiis.p, bytes, hipMemcpyDeviceToHost, pb.s));
+        pb.down.mark(k, pb.s);
+        HIPCHK(hipStreamSynchronize(pb.s));
+        for (int b : back) unpack(b, static_cast<const MdiisDev*>(h)[b]);
+      }
+      if (any_rebase) {
+        std::vector<double> rinf, r2;
+        rebase(pb, cf, rebase_mask, true, rinf, r2);
+        after_rebase(rebase_mask, rinf, r2, false);
+        for (int b = 0; b < B; ++b)
+          if (rebase_mask[b]) dev_dirty.push_back(b);
+      }
+      continue;
     }
+    for (int b = 0; b < B; ++b) make_desc(b);
     evaluate(pb, fp32, sd);
     download_out(pb, out);
     HIPCHK(hipGetLastError());
@@ -2473,7 +2683,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
       any = true;
     }
     if (any) {
-      upload_sd(pb, sd);
+      upload_sd(pb, sd, pb.d_sd.p);
       using T = double;
       constexpr int LN = y_lines<T>();
       const dim3 gx(resid_grid_x(g), m, B);
@@ -2649,7 +2859,7 @@ int run_impl(const Inputs& in, rism3d_result* res, std::vector<double>* t_init,
       sd[b].mode = 3;
       sd[b].active = s.status == RISM3D_OK ? 1 : 0;
     }
-    upload_sd(pb, sd);
+    upload_sd(pb, sd, pb.d_sd.p);
     using T = double;
     constexpr int LN = y_lines<T>();
     const dim3 gx(resid_grid_x(g), m, B);
