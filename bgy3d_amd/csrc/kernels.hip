@@ -1490,7 +1490,9 @@ __device__ __forceinline__ double lane_bcast(double v, int lane) {
 // and the descriptor of the next pass (the combination t = u_newest -
 // sum_i (c_0 + .. + c_i) (u_{i+1} - u_i) over the BF16 differences, the
 // slot of the new entry, the FP8 scales).  A solute that wants a rebase or
-// failed is left inactive for the host.  The DIIS system is solved as
+// failed is left inactive for the host; with hold, one whose residual fell
+// 1e3 below its base's goes on and is reported ready (the host groups
+// rebases).  The DIIS system is solved as
 // diis_coefficients solves it on the host, to the bit: LU with partial
 // pivoting (first largest pivot), one step of refinement against the
 // unfactored matrix, substitutions by columns in the host's order of
@@ -1536,9 +1538,11 @@ mdiis_step(const double* __restrict__ out, MdiisDev* __restrict__ mst,
     } else {
       if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; }
       else if (rinf <= (s.only32 ? 1e-3 : 1e-2 * s.rb_rinf)) { s.stall32++; }
-      if ((!s.only32 && rinf <= 1e-3 * s.rb_rinf) || rinf <= s.tol_half || s.stall32 >= 12 ||
-          s.iters32 >= s.max_iter)
+      const bool ready = !s.only32 && rinf <= 1e-3 * s.rb_rinf;
+      if (rinf <= s.tol_half || s.stall32 >= 12 || s.iters32 >= s.max_iter || (ready && !s.hold))
         flag = 1;
+      else if (ready)
+        flag = 4;
       s.sbase[sn] = s.base;
       s.order[nd0] = sn;
       s.n = nd0 + 1;
@@ -1685,7 +1689,7 @@ mdiis_step(const double* __restrict__ out, MdiisDev* __restrict__ mst,
     s.coef[0] = 1.0;
   }
   __syncthreads();
-  if (flag == 0) {
+  if (flag == 0 || flag == 4) {
     // the next pass: t = u_newest - sum_i (c_0 + .. + c_i) (u_{i+1} - u_i),
     // the new r and u in a free slot, else in the oldest one (dropped)
     const int n = s.n, D = s.D;
@@ -1729,7 +1733,7 @@ mdiis_step(const double* __restrict__ out, MdiisDev* __restrict__ mst,
     MdiisStatus& st = status[b];
     st.flag = flag;
     st.iters32 = s.iters32;
-    st.hist = flag == 0 ? (s.n > s.ndot ? s.n : s.ndot) : 0;
+    st.hist = (flag == 0 || flag == 4) ? (s.n > s.ndot ? s.n : s.ndot) : 0;
     st.resid = rinf;
     st.r2 = r2;
     __threadfence_system();

@@ -587,6 +587,7 @@ struct SoluteState {
   double best32 = INFINITY;   // FP32: best max|F| since the last rebase
   int stall32 = 0;
   double rb_rinf = INFINITY;  // FP32: max|F(t_b)| at the last rebase (FP64)
+  int ready_at = -1;          // FP32: iters32 when ready for a rebase (-1: not)
   int polish = 0;             // FP64 steps taken after the FP32 phase
   double resid = INFINITY, best = INFINITY;
   int since_best = 0;
@@ -648,6 +649,10 @@ struct Problem {
   // the host reads after each pass.  RISM3D_DEVICE_MDIIS=0 keeps the step
   // on the host.
   int dev_mdiis = env_int("RISM3D_DEVICE_MDIIS", 1);
+  // FP32 phase: a solute whose residual fell 1e3 below its base's waits for
+  // a rebase, iterating on, until another solute needs one or for at most
+  // this many passes, so that rebases run together (0: each at once)
+  int rebase_wait = env_int("RISM3D_REBASE_WAIT", 8);
   DevBuf<SoluteDesc> d_sd32;
   DevBuf<MdiisDev> d_mdiis;
   struct PinnedStatus {
@@ -1061,6 +1066,7 @@ double lam32(const Problem& pb, const Config& cf) { return pb.lam32 > 0 ? pb.lam
 
 void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
   auto& s = pb.st[b];
+  s.ready_at = -1;
   const int64_t mP = pb.mP;
   const int D = pb.D;
   const int ch = (int)std::min<int64_t>(2048, (mP + NT - 1) / NT);
@@ -1238,6 +1244,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
     m.stall32 = s.stall32;
     m.iters = s.iters;
     m.iters32 = s.iters32;
+    m.hold = pb.rebase_wait > 0;
     for (int i = 0; i < m.n; ++i) {
       m.order[i] = s.order[i];
       m.coef[i] = s.coef[i];
@@ -1314,6 +1321,27 @@ void iterate_batch(Problem& pb, const Config& cf) {
       }
     }
     HIPCHK(hipGetLastError());
+  };
+  // Rebases together: once a solute must rebase (or one ready has waited
+  // rebase_wait passes), every solute ready for one joins it.
+  auto group_rebases = [&](std::vector<int>& mask, bool& any_rebase, std::vector<int>* back) {
+    if (pb.rebase_wait <= 0) return;
+    bool go = any_rebase;
+    for (int b = 0; b < B; ++b) {
+      const auto& s = pb.st[b];
+      if (s.active && s.phase32 && !mask[b] && s.ready_at >= 0 &&
+          s.iters32 - s.ready_at >= pb.rebase_wait)
+        go = true;
+    }
+    if (!go) return;
+    for (int b = 0; b < B; ++b) {
+      const auto& s = pb.st[b];
+      if (s.active && s.phase32 && !mask[b] && s.ready_at >= 0) {
+        mask[b] = 1;
+        any_rebase = true;
+        if (back) back->push_back(b);
+      }
+    }
   };
   for (;;) {
     bool any = false, any32 = false;
@@ -1447,8 +1475,11 @@ void iterate_batch(Problem& pb, const Config& cf) {
           rebase_mask[b] = 1;
           any_rebase = true;
           back.push_back(b);
+        } else if (q.flag == 4 && s.ready_at < 0) {
+          s.ready_at = q.iters32;
         }
       }
+      group_rebases(rebase_mask, any_rebase, &back);
       if (!back.empty()) {  // their state back to the host
         const size_t bytes = (size_t)B * sizeof(MdiisDev);
         int k = 0;
@@ -1501,10 +1532,13 @@ void iterate_batch(Problem& pb, const Config& cf) {
         if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; }
         else if (rinf <= (only32 ? 1e-3 : 1e-2 * s.rb_rinf)) { s.stall32++; }
         // The new entry still enters the history (the rebase moves it).
-        if ((!only32 && rinf <= 1e-3 * s.rb_rinf) || rinf <= 0.5 * s.tol_mixed ||
-            s.stall32 >= 12 || s.iters32 >= cf.max_iter) {
+        const bool ready = !only32 && rinf <= 1e-3 * s.rb_rinf;
+        if (rinf <= 0.5 * s.tol_mixed || s.stall32 >= 12 || s.iters32 >= cf.max_iter ||
+            (ready && pb.rebase_wait <= 0)) {
           rebase_mask[b] = 1;
           any_rebase = true;
+        } else if (ready && s.ready_at < 0) {
+          s.ready_at = s.iters32;
         }
         s.sbase[s.slot_new] = s.base;
       } else {
@@ -1556,6 +1590,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
         s.coef.assign(1, 1.0);
       }
     }
+    if (fp32) group_rebases(rebase_mask, any_rebase, nullptr);
     if (any_rebase) {
       std::vector<double> rinf, r2;
       rebase(pb, cf, rebase_mask, true, rinf, r2);

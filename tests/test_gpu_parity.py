@@ -356,30 +356,36 @@ def test_graph_replay_matches_eager(chi_w):
 
 
 @pytest.mark.gpu
-def test_device_mdiis_matches_host(chi_w):
+@pytest.mark.parametrize("wait", ["0", "3"])
+def test_device_mdiis_matches_host(chi_w, wait):
     """The FP32 phase's MDIIS step on the device (mdiis_step, the default)
     against the same step on the host (RISM3D_DEVICE_MDIIS=0): the device
     solves the DIIS system with the host's operations in the host's order,
     so a mixed batch (hundreds of passes, rebases, restarts of the history)
-    gives bit-identical mu, g and iteration counts."""
-    print("case device vs host MDIIS", flush=True)
+    gives bit-identical mu, g and iteration counts, with rebases one by one
+    (RISM3D_REBASE_WAIT=0) or grouped (3: solutes ready for a rebase wait up
+    to 3 passes for one another, iterating on)."""
+    print(f"case device vs host MDIIS, rebase wait {wait}", flush=True)
     import os
     from bgy3d_amd import rism3d as R
     from bgy3d_amd import synthetic as S
     sols = [S.solute(s) for s in range(6)]
     st = dict(README, N=32, L=8.0, rmax=40.0, nrad=1536, closure="KH",
               **{"norm-tol": 1e-10, "precision": "mixed"})
-    old = os.environ.get("RISM3D_DEVICE_MDIIS")
+    keys = ("RISM3D_DEVICE_MDIIS", "RISM3D_REBASE_WAIT")
+    old = {k: os.environ.get(k) for k in keys}
     try:
+        os.environ["RISM3D_REBASE_WAIT"] = wait
         os.environ["RISM3D_DEVICE_MDIIS"] = "1"
         a = R.batch_solve(sols, "water, PR-SPC/E", st, chi_w, want_g=True)
         os.environ["RISM3D_DEVICE_MDIIS"] = "0"
         b = R.batch_solve(sols, "water, PR-SPC/E", st, chi_w, want_g=True)
     finally:
-        if old is None:
-            os.environ.pop("RISM3D_DEVICE_MDIIS", None)
-        else:
-            os.environ["RISM3D_DEVICE_MDIIS"] = old
+        for k in keys:
+            if old[k] is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = old[k]
     for x, y in zip(a, b):
         print(f"  {x['status']} {x['free-energy']!r} {y['free-energy']!r} iters {x['iterations']} {y['iterations']}",
               flush=True)
