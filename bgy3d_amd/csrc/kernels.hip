@@ -1491,7 +1491,8 @@ __device__ __forceinline__ double lane_bcast(double v, int lane) {
 // sum_i (c_0 + .. + c_i) (u_{i+1} - u_i) over the BF16 differences, the
 // slot of the new entry, the FP8 scales).  A solute that wants a rebase or
 // failed is left inactive for the host; with hold, one whose residual fell
-// 1e3 below its base's goes on and is reported ready (the host groups
+// 1e3 below its base's goes on and is reported ready, and one that reached
+// the phase's tolerance is left inactive as parked (the host groups
 // rebases).  The DIIS system is solved as
 // diis_coefficients solves it on the host, to the bit: LU with partial
 // pivoting (first largest pivot), one step of refinement against the
@@ -1538,10 +1539,15 @@ mdiis_step(const double* __restrict__ out, MdiisDev* __restrict__ mst,
     } else {
       if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; }
       else if (rinf <= (s.only32 ? 1e-3 : 1e-2 * s.rb_rinf)) { s.stall32++; }
-      const bool ready = !s.only32 && rinf <= 1e-3 * s.rb_rinf;
-      if (rinf <= s.tol_half || s.stall32 >= 12 || s.iters32 >= s.max_iter || (ready && !s.hold))
+      // with hold: the 1e3 drop only marks the solute ready, and one that
+      // reached the phase's tolerance waits without iterating (parked)
+      const bool drop = !s.only32 && rinf <= 1e-3 * s.rb_rinf, conv = rinf <= s.tol_half;
+      if (s.stall32 >= 12 || s.iters32 >= s.max_iter || (!s.hold && (drop || conv)) ||
+          (s.only32 && conv))
         flag = 1;
-      else if (ready)
+      else if (conv)
+        flag = 5;
+      else if (drop)
         flag = 4;
       s.sbase[sn] = s.base;
       s.order[nd0] = sn;

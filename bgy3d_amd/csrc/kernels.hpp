@@ -81,7 +81,8 @@ struct MdiisDev {
   int slot_new;           // slot receiving the pending r and u
   int ndot;               // entries the pending r is dotted with (dotted)
   int since_best, stall32, iters, iters32;
-  int hold;               // the 1e3 drop only marks the solute ready (flag 4)
+  int hold;               // the 1e3 drop only marks the solute ready (flag 4),
+                          // the phase's tolerance parks it (flag 5)
   int order[MAX_DIIS];
   int dotted[MAX_DIIS];
   int sbase[MAX_DIIS];    // base of each slot's entry
@@ -95,7 +96,8 @@ static_assert(sizeof(MdiisDev) % 8 == 0, "MdiisDev is copied in 8-byte words");
 // What mdiis_step reports to the host per solute and pass (pinned memory).
 struct MdiisStatus {
   int flag;               // 0 next pass prepared, 1 rebase wanted, 2 not
-                          // finite, 4 ready for a rebase, next pass prepared
+                          // finite, 4 ready for a rebase, next pass prepared,
+                          // 5 at the phase's tolerance: parked until a rebase
   int iters, iters32;
   int hist;               // flag 0: history slots of the next pass, max(ncomb, ndot)
   double resid, r2;       // max|F|, ||F||^2 of the pass

@@ -587,7 +587,8 @@ struct SoluteState {
   double best32 = INFINITY;   // FP32: best max|F| since the last rebase
   int stall32 = 0;
   double rb_rinf = INFINITY;  // FP32: max|F(t_b)| at the last rebase (FP64)
-  int ready_at = -1;          // FP32: iters32 when ready for a rebase (-1: not)
+  int ready_at = -1;          // FP32: pass of the batch when ready for a rebase (-1: not)
+  int parked = 0;             // FP32: at mixed_tol / 2, waiting for a rebase without passes
   int polish = 0;             // FP64 steps taken after the FP32 phase
   double resid = INFINITY, best = INFINITY;
   int since_best = 0;
@@ -1067,6 +1068,7 @@ double lam32(const Problem& pb, const Config& cf) { return pb.lam32 > 0 ? pb.lam
 void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
   auto& s = pb.st[b];
   s.ready_at = -1;
+  s.parked = 0;
   const int64_t mP = pb.mP;
   const int D = pb.D;
   const int ch = (int)std::min<int64_t>(2048, (mP + NT - 1) / NT);
@@ -1173,6 +1175,8 @@ void iterate_batch(Problem& pb, const Config& cf) {
     for (int b = 0; b < B; ++b) {
       if (!mask[b]) continue;
       auto& s = pb.st[b];
+      s.parked = 0;
+      s.ready_at = -1;
       s.iters++;
       s.resid = rinf[b];
       if (cf.verbose)
@@ -1301,7 +1305,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
   auto wait_status = [&]() {
     std::vector<int> expect(B, 0);
     for (int b = 0; b < B; ++b)
-      if (pb.st[b].active && pb.st[b].phase32) expect[b] = pb.st[b].iters + 1;
+      if (pb.st[b].active && pb.st[b].phase32 && !pb.st[b].parked) expect[b] = pb.st[b].iters + 1;
     auto arrived = [&]() {
       for (int b = 0; b < B; ++b)
         if (expect[b] && __atomic_load_n(&pb.h_status.p[b].iters, __ATOMIC_ACQUIRE) != expect[b])
@@ -1322,18 +1326,21 @@ void iterate_batch(Problem& pb, const Config& cf) {
     }
     HIPCHK(hipGetLastError());
   };
-  // Rebases together: once a solute must rebase (or one ready has waited
-  // rebase_wait passes), every solute ready for one joins it.
+  // Rebases together: once a solute must rebase, or one ready has waited
+  // rebase_wait passes, or every solute of the phase is ready, every solute
+  // ready for one joins it.
+  int npass32 = 0;  // FP32 passes of the batch so far
   auto group_rebases = [&](std::vector<int>& mask, bool& any_rebase, std::vector<int>* back) {
     if (pb.rebase_wait <= 0) return;
-    bool go = any_rebase;
+    bool go = any_rebase, all = true, some = false;
     for (int b = 0; b < B; ++b) {
       const auto& s = pb.st[b];
-      if (s.active && s.phase32 && !mask[b] && s.ready_at >= 0 &&
-          s.iters32 - s.ready_at >= pb.rebase_wait)
-        go = true;
+      if (!s.active || !s.phase32 || mask[b]) continue;
+      if (s.ready_at < 0) { all = false; continue; }
+      some = true;
+      if (npass32 - s.ready_at >= pb.rebase_wait) go = true;
     }
-    if (!go) return;
+    if (!go && !(all && some)) return;
     for (int b = 0; b < B; ++b) {
       const auto& s = pb.st[b];
       if (s.active && s.phase32 && !mask[b] && s.ready_at >= 0) {
@@ -1360,7 +1367,7 @@ void iterate_batch(Problem& pb, const Config& cf) {
       auto& s = pb.st[b];
       SoluteDesc& d = sd[b];
       std::memset(&d, 0, sizeof d);
-      d.active = s.active && (s.phase32 != 0) == fp32;
+      d.active = s.active && !s.parked && (s.phase32 != 0) == fp32;
       if (!d.active) return;
       auto slotU = [&](int q) { return (void*)((char*)s.U.p + (size_t)q * mP * sz); };
       const size_t szr = fp32 && pb.r8 ? 1 : (fp32 && pb.r16 ? 2 : sz);  // FP8 / BF16 history
@@ -1451,14 +1458,16 @@ void iterate_batch(Problem& pb, const Config& cf) {
       }
       int hist = 0;
       for (int b = 0; b < B; ++b)
-        if (pb.st[b].active && pb.st[b].phase32) hist = std::max(hist, next_hist[b]);
+        if (pb.st[b].active && pb.st[b].phase32 && !pb.st[b].parked)
+          hist = std::max(hist, next_hist[b]);
       evaluate(pb, true, sd, 1, hist);
       wait_status();
+      ++npass32;
       std::vector<int> rebase_mask(B, 0), back;
       bool any_rebase = false;
       for (int b = 0; b < B; ++b) {
         auto& s = pb.st[b];
-        if (!(s.active && s.phase32)) continue;
+        if (!(s.active && s.phase32) || s.parked) continue;
         const MdiisStatus& q = pb.h_status.p[b];
         s.iters = q.iters;
         s.iters32 = q.iters32;
@@ -1475,8 +1484,9 @@ void iterate_batch(Problem& pb, const Config& cf) {
           rebase_mask[b] = 1;
           any_rebase = true;
           back.push_back(b);
-        } else if (q.flag == 4 && s.ready_at < 0) {
-          s.ready_at = q.iters32;
+        } else if (q.flag == 4 || q.flag == 5) {
+          if (s.ready_at < 0) s.ready_at = npass32;
+          s.parked = q.flag == 5;
         }
       }
       group_rebases(rebase_mask, any_rebase, &back);
@@ -1532,13 +1542,15 @@ void iterate_batch(Problem& pb, const Config& cf) {
         if (rinf < 0.7 * s.best32) { s.best32 = rinf; s.stall32 = 0; }
         else if (rinf <= (only32 ? 1e-3 : 1e-2 * s.rb_rinf)) { s.stall32++; }
         // The new entry still enters the history (the rebase moves it).
-        const bool ready = !only32 && rinf <= 1e-3 * s.rb_rinf;
-        if (rinf <= 0.5 * s.tol_mixed || s.stall32 >= 12 || s.iters32 >= cf.max_iter ||
-            (ready && pb.rebase_wait <= 0)) {
+        // ready (with a rebase wait): the 1e3 drop, or mixed_tol / 2 reached
+        const bool drop = !only32 && rinf <= 1e-3 * s.rb_rinf, conv = rinf <= 0.5 * s.tol_mixed;
+        if (s.stall32 >= 12 || s.iters32 >= cf.max_iter || (pb.rebase_wait <= 0 && (drop || conv)) ||
+            (only32 && conv)) {
           rebase_mask[b] = 1;
           any_rebase = true;
-        } else if (ready && s.ready_at < 0) {
-          s.ready_at = s.iters32;
+        } else if (drop || conv) {
+          if (s.ready_at < 0) s.ready_at = npass32 + 1;
+          s.parked = conv;
         }
         s.sbase[s.slot_new] = s.base;
       } else {
@@ -1590,7 +1602,10 @@ void iterate_batch(Problem& pb, const Config& cf) {
         s.coef.assign(1, 1.0);
       }
     }
-    if (fp32) group_rebases(rebase_mask, any_rebase, nullptr);
+    if (fp32) {
+      ++npass32;
+      group_rebases(rebase_mask, any_rebase, nullptr);
+    }
     if (any_rebase) {
       std::vector<double> rinf, r2;
       rebase(pb, cf, rebase_mask, true, rinf, r2);
