@@ -97,6 +97,15 @@ template <typename T> __device__ __forceinline__ void stc(cpx<T>* p, cpx<T> c) {
   x[1] = c.im;
   *(GV*)p = x;
 }
+// base[i] for a base the same in every lane and an element index below
+// 2^32 bytes from it: the byte offset stays 32-bit, so the access takes the
+// scalar-base form instead of a 64-bit address per lane.
+template <typename X> __device__ __forceinline__ X* at32(X* base, uint32_t i) {
+  return reinterpret_cast<X*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(X));
+}
+template <typename X> __device__ __forceinline__ const X* at32(const X* base, uint32_t i) {
+  return reinterpret_cast<const X*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(X));
+}
 template <typename X> __device__ __forceinline__ X gld(const X* p) {  // scalars
   typedef __attribute__((address_space(1))) const X GX;
   return *(GX*)p;
@@ -763,11 +772,14 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
   // kx0 + l of row y.
   const int y = ky;
   const int kx0 = NYQ ? g.NX - 1 : tile * LINES;
-  const int64_t zs = (int64_t)g.N1 * g.NH;  // z stride (Fs, chi, tau)
-  const int64_t ls = NYQ ? g.NH : 1;        // line stride
-  const int64_t base = (int64_t)y * g.NH + kx0;
+  // element offsets below 2^32 bytes from per-workgroup bases (at32): the
+  // tables and Fs from their starts, W from its site's
+  const int zs = g.N1 * g.NH;         // z stride (Fs, chi, tau)
+  const int ls = NYQ ? g.NH : 1;      // line stride
+  const int base = y * g.NH + kx0;
   // W: the same index unless ylay (y-major rows, compact Nyquist column)
-  int64_t wz = zs, wl = ls, wsite = g.K;
+  int wz = zs, wl = ls;
+  int64_t wsite = g.K;
   cpx<T>* W = static_cast<cpx<T>*>(d.W) + base;
   if (g.ylay) {
     if (NYQ) {
@@ -793,7 +805,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
       const int e = tid + p * NTZ;
       if (PPT * NTZ == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
-        c[s][p] = (nl == LINES || l < nl) ? ldc(W + s * wsite + z * wz + l * wl)
+        c[s][p] = (nl == LINES || l < nl) ? ldc(at32(W + s * wsite, (uint32_t)(z * wz + l * wl)))
                                           : cpx<T>{T(0), T(0)};
       }
     }
@@ -814,7 +826,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
         for (int p = 0; p < PPT; ++p) {
           const int e = tid + p * NTZ;
           const int z = e / LINES, l = e - z * LINES;
-          if (l < nl) stc(Ws + z * wz + l * wl, c[s][p]);
+          if (l < nl) stc(at32(Ws, (uint32_t)(z * wz + l * wl)), c[s][p]);
         }
       }
       continue;
@@ -836,7 +848,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
       if (PPT * NTZ == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
         c[s][p] = lds[e];
-        if (d.mode == 3 && l < nl) stc(Ws + z * wz + l * wl, c[s][p]);  // forward spectrum only
+        if (d.mode == 3 && l < nl) stc(at32(Ws, (uint32_t)(z * wz + l * wl)), c[s][p]);  // forward spectrum only
       }
     }
     __syncthreads();
@@ -850,9 +862,9 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
     if (d.mode == 4 || !(PPT * NTZ == PTS || e < PTS)) continue;
     const int z = e / LINES, l = e - z * LINES;
     if (l >= nl) continue;
-    const int64_t kk = base + z * zs + l * ls;  // k-grid index of this point
+    const uint32_t kk = (uint32_t)(base + z * zs + l * ls);  // k-grid index of this point
     cpx<T> fs = {T(0), T(0)};
-    if (!linear) fs = ldc(Fs + fs_index<T>(g, z, NYQ ? y + l : y, kx0 + (NYQ ? 0 : l)));
+    if (!linear) fs = ldc(at32(Fs, (uint32_t)fs_index<T>(g, z, NYQ ? y + l : y, kx0 + (NYQ ? 0 : l))));
     cpx<T> uc = {T(0), T(0)};
     if (!g.renorm && !linear) {
       // u_c(k) = 4 pi eps0^-1 exp(-k^2/4G^2)/k^2 F(k) s(k), 0 at k = 0
@@ -870,7 +882,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
     }
     T chi[MS * (MS + 1) / 2];
 #pragma unroll
-    for (int q = 0; q < MS * (MS + 1) / 2; ++q) chi[q] = gld(chi3 + q * g.K + kk);
+    for (int q = 0; q < MS * (MS + 1) / 2; ++q) chi[q] = gld(at32(chi3 + q * g.K, kk));
     cpx<T> out[MS];
 #pragma unroll
     for (int i = 0; i < MS; ++i) {
@@ -886,7 +898,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
         out[i] = cscale(acc, T(g.a_scale));
       } else if (g.renorm) {
         // (h^3/L^3) sum_j chi_ij C_j - (beta/eps0/L^3) tau_i F s  (hnc3d.c:1245-1268)
-        const T tau = gld(tau3 + i * g.K + kk) * T(g.tau_scale);
+        const T tau = gld(at32(tau3 + i * g.K, kk)) * T(g.tau_scale);
         out[i] = csub(cscale(acc, T(g.a_scale)), cscale(fs, tau));
       } else {
         out[i] = cscale(csub(acc, cscale(uc, T(g.beta * g.q[i]))), T(g.inv_L3));
@@ -904,7 +916,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
       for (int p = 0; p < PPT; ++p) {
         const int e = tid + p * NTZ;
         const int z = e / LINES, l = e - z * LINES;
-        if (l < nl) stc(Ws + z * wz + l * wl, c[s][p]);
+        if (l < nl) stc(at32(Ws, (uint32_t)(z * wz + l * wl)), c[s][p]);
       }
       continue;
     }
@@ -920,7 +932,7 @@ zstar_reg(const SoluteDesc* __restrict__ sd, GridArgs g, const cpx<T>* __restric
       const int e = tid + p * NTZ;
       if (PPT * NTZ == PTS || e < PTS) {
         const int z = e / LINES, l = e - z * LINES;
-        if (l < nl) stc(Ws + z * wz + l * wl, lds[e]);
+        if (l < nl) stc(at32(Ws, (uint32_t)(z * wz + l * wl)), lds[e]);
       }
     }
     __syncthreads();
