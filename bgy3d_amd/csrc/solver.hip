@@ -1129,8 +1129,10 @@ void rebase_start(Problem& pb, const Config& cf, int b, double r2, bool seed) {
 // Runs the fixed-point iteration for every solute of the batch.  MDIIS
 // (Pulay) over the history u_j = t_j + lam r_j: t = sum c_j u_j with c
 // minimising |sum c_j r_j|, sum c_j = 1.  Depth 1 is Picard, t += lam r
-// (bgy3d_snes_picard, bgy3d-snes.c:348-384).  One host synchronisation per
-// iteration, for the norms and the Gram row.
+// (bgy3d_snes_picard, bgy3d-snes.c:348-384).  The FP32 phase of a mixed
+// solve takes its MDIIS step on the device (mdiis_step) and the host polls
+// a pinned status per pass; otherwise the host reads the norms and the
+// Gram row back once per iteration and takes the step itself.
 void iterate_batch(Problem& pb, const Config& cf) {
   const int B = pb.B;
   const int64_t mP = pb.mP;
