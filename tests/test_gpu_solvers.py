@@ -50,20 +50,44 @@ def test_jacobian_vs_oracle(closure, solute, flag, chi_w):
     assert err < 1e-9
 
 
+_ORACLE_ROOTS = {}
+
+
+def _oracle_root(closure, solute, chi_w):
+    """The oracle's root of the same problem (max|F| <= 1e-11), once per
+    closure and solute."""
+    from bgy3d_amd.solutes import find_molecule
+    key = (closure, solute)
+    if key not in _ORACLE_ROOTS:
+        pd = O.Problem.from_cli(8.0, 32, README["beta"], README["rho"], closure=C.NAMES[closure],
+                                rmax=40.0, nrad=1536)
+        _ORACLE_ROOTS[key] = O.solute_solve(pd, find_molecule(W), find_molecule(solute), chi_w,
+                                            atol=1e-11)
+    return _ORACLE_ROOTS[key]
+
+
 @pytest.mark.parametrize("solver", ["newton", "trial", "jager", "picard"])
 @pytest.mark.parametrize("closure,solute", [("HNC", W), ("KH", "Na+, JC08, SPC/E")])
 def test_reference_solvers_reach_the_mdiis_root(solver, closure, solute, chi_w):
+    """Newton-GMRES, trial, Jager and Picard reach the MDIIS root of the GPU
+    (|dmu| 1e-9, max|dg| 1e-8) and the oracle's root of the same problem
+    (|dmu| 1e-8, max|dg| 1e-7)."""
     print(f"case solver {solver} {closure} {solute}", flush=True)
     from bgy3d_amd import rism3d as R
     s = _settings(closure, **{"norm-tol": 1e-11, "max-iter": 20000})
     ref = R.hnc3d_run_solute(solute, W, s, chi=chi_w)
+    orc = _oracle_root(closure, solute, chi_w)
     lam = {"picard": 0.3, "jager": 0.1}.get(solver, 0.02)
     out = R.hnc3d_run_solute(solute, W, dict(s, **{"snes-solver": solver, "lambda": lam}),
                              chi=chi_w)
     dmu = abs(out["free-energy"] - ref["free-energy"])
     dg = np.max(np.abs(out["GUV"] - ref["GUV"]))
+    omu = abs(out["free-energy"] - orc["mu"])
+    og = np.max(np.abs(out["GUV"] - orc["g"]))
     print(f"  {out['status']} passes {out['iterations']} (mdiis {ref['iterations']}) "
-          f"|dmu| {dmu:.2e} max|dg| {dg:.2e}", flush=True)
+          f"|dmu| {dmu:.2e} max|dg| {dg:.2e}; vs oracle |dmu| {omu:.2e} max|dg| {og:.2e}",
+          flush=True)
     assert out["status"] == "OK"
     assert dmu < 1e-9 and dg < 1e-8
+    assert omu < 1e-8 and og < 1e-7
 
