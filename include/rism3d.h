@@ -102,10 +102,9 @@ enum rism3d_precision {
                                every rebase, then polish_iters FP64 steps;
                                no FP32 floor.  The FP32 MDIIS history is
                                kept as the newest u in FP32, BF16
-                               differences of u and FP8 residuals; its
-                               step runs on the device between passes,
-                               and solutes ready for a rebase wait for
-                               one another (grouped FP64 passes)         */
+                               differences of u and FP8 residuals;
+                               solutes ready for a rebase wait for one
+                               another (grouped FP64 passes)             */
   RISM3D_FP32 = 2           /* FP32 passes only, around the starting
                                point: they stop at the FP32 floor (no
                                rebase, no polish); the FP64 residual there

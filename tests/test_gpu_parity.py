@@ -324,29 +324,35 @@ def test_continuation_hnc(chi_w):
     assert m["status"] == "OK" and dmu < 0.01
 
 
-def test_graph_replay_matches_eager(chi_w):
+@pytest.mark.parametrize("dev", ["0", "1"])
+def test_graph_replay_matches_eager(chi_w, dev):
     """The residual chain replayed as HIP graphs (default) against the same
     kernels launched one by one (RISM3D_GRAPHS=0): a mixed-precision batch
     (FP32 and FP64 graphs, hundreds of replays with uploads, read-backs and
-    rebases between them) gives bit-identical mu and iteration counts."""
-    print("case graphs vs eager", flush=True)
+    rebases between them) gives bit-identical mu and iteration counts, with
+    the MDIIS step on the host (default) or on the device (the FP32 graphs
+    then end in mdiis_step)."""
+    print(f"case graphs vs eager, device MDIIS {dev}", flush=True)
     import os
     from bgy3d_amd import rism3d as R
     from bgy3d_amd import synthetic as S
     sols = [S.solute(s) for s in range(4)]
     st = dict(README, N=32, L=8.0, rmax=40.0, nrad=1536, closure="KH",
               **{"norm-tol": 1e-10, "precision": "mixed"})
-    old = os.environ.get("RISM3D_GRAPHS")
+    keys = ("RISM3D_GRAPHS", "RISM3D_DEVICE_MDIIS")
+    old = {k: os.environ.get(k) for k in keys}
     try:
+        os.environ["RISM3D_DEVICE_MDIIS"] = dev
         os.environ["RISM3D_GRAPHS"] = "1"
         a = R.batch_solve(sols, "water, PR-SPC/E", st, chi_w)
         os.environ["RISM3D_GRAPHS"] = "0"
         b = R.batch_solve(sols, "water, PR-SPC/E", st, chi_w)
     finally:
-        if old is None:
-            os.environ.pop("RISM3D_GRAPHS", None)
-        else:
-            os.environ["RISM3D_GRAPHS"] = old
+        for k in keys:
+            if old[k] is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = old[k]
     for x, y in zip(a, b):
         print(f"  {x['status']} {x['free-energy']!r} {y['free-energy']!r} iters {x['iterations']} {y['iterations']}",
               flush=True)
@@ -358,9 +364,10 @@ def test_graph_replay_matches_eager(chi_w):
 @pytest.mark.gpu
 @pytest.mark.parametrize("wait", ["0", "3"])
 def test_device_mdiis_matches_host(chi_w, wait):
-    """The FP32 phase's MDIIS step on the device (mdiis_step, the default)
-    against the same step on the host (RISM3D_DEVICE_MDIIS=0): the device
-    solves the DIIS system with the host's operations in the host's order,
+    """The FP32 phase's MDIIS step on the device (mdiis_step,
+    RISM3D_DEVICE_MDIIS=1) against the same step on the host (the default):
+    the device solves the DIIS system with the host's operations in the
+    host's order,
     so a mixed batch (hundreds of passes, rebases, restarts of the history)
     gives bit-identical mu, g and iteration counts, with rebases one by one
     (RISM3D_REBASE_WAIT=0) or grouped (3: solutes ready for a rebase wait up
